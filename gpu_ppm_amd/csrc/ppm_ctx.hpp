@@ -1,0 +1,122 @@
+// The context behind the opaque ppm_ctx handle (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ppm_internal.hpp"
+#include "ppm_tables.hpp"
+
+namespace ppm {
+
+// Device buffer that only grows (no per-call hipMalloc once warm).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+// Pinned host buffer (hipHostMalloc) for asynchronous uploads.
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+struct Workspace {
+  DevBuf spots, tasks, desc, ph, fine, pts, energies, partial, scale, out_flux, out_counts;
+  void release_all() {
+    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &partial, &scale, &out_flux,
+                      &out_counts})
+      b->release();
+  }
+};
+
+// Buffers of the batched path (ppm_batch.cpp).
+struct BatchWorkspace {
+  DevBuf spots, tasks, desc, ph, fine, pts, energies, items, flux_acc, scale, counter, rowmap;
+  void release_all() {
+    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &items, &flux_acc, &scale, &counter,
+                      &rowmap})
+      b->release();
+  }
+};
+
+struct Response {
+  std::string setting;
+  int n_ch = 0, n_e = 0;
+  std::vector<double> e_obs, rsp;
+  DevBuf buf;
+  const uint16_t* d_hi = nullptr;  // BF16 high part of FP32(rsp)
+  const uint16_t* d_lo = nullptr;  // BF16 of the remainder
+};
+
+struct ServeState;  // ppm_serve.cpp
+
+inline uint16_t bf16_round(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+inline float bf16_value(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+}  // namespace ppm
+
+struct ppm_ctx {
+  int device = 0;
+  int max_lds = 65536;
+  size_t budget = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_i0 = nullptr, ev_i1 = nullptr, ev_staging = nullptr;
+  std::string err;
+  std::mutex exec_mu;  // one evaluation on the device at a time
+
+  ppm::LensingHost lens;
+  ppm::DevLensing L{};
+  ppm::DevBuf d_lens;
+  ppm::DevNsx N{};
+  ppm::DevBuf d_nsx, d_mu;
+  std::vector<double> nsx_mu;
+  std::vector<ppm::Response> resp;
+
+  ppm::Workspace ws;
+  ppm::BatchWorkspace batch;
+  ppm::HostBuf staging;
+
+  ppm::ServeState* serve = nullptr;
+};
+
+namespace ppm {
+
+ppm_status fail(ppm_ctx* c, ppm_status st, const std::string& msg);
+ppm_status hip_fail(ppm_ctx* c, int e, const char* what);
+ppm_status validate(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, int n_e_resolved);
+void spot_params(const ppm_ctx* c, const ppm_star& st, const ppm_spot& sp, int n_side, SpotParams& P);
+void candidate_rings(const SpotParams& P, const ppm_spot& sp, int n_side, std::vector<int>& rings);
+void phase_points(const ppm_grid& g, std::vector<PhasePoint>& pts);
+ppm_status resolve_energies(ppm_ctx* c, const ppm_grid& g, std::vector<double>& e);
+ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
+                    double* d_counts, ppm_stats* stats);
+
+void serve_init(ppm_ctx* c);
+void serve_shutdown(ppm_ctx* c);
+
+}  // namespace ppm

@@ -1,0 +1,793 @@
+// HIP kernels of the pulse-profile path for gfx950 (MI355X).
+//
+//   k_ring_setup      one workgroup per (spot, candidate ring): FP64 patch
+//                     membership compacted into runs by wavefront ballot and
+//                     prefix scan; FP64 fine-phase table of flux/I, redshift,
+//                     cos(sigma') and time-delayed phase; ring-skip rule, gap
+//                     fill, closure; NSX (logT, log g) bracket.
+//   k_integral_plain  one workgroup per (ring, 64-energy chunk): NSX slice
+//                     blended into LDS, per-patch set-up in FP64 by one lane
+//                     each, FP32 cubic-Lagrange evaluation with one energy per
+//                     lane, per-bin Kahan accumulation in registers.
+//   k_reduce          fixed-order FP64 sum of the per-ring partials.
+//   k_fold            response fold on MFMA (split BF16, FP32 accumulate).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ppm_internal.hpp"
+
+namespace ppm {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// HEALPix ring i of N_side n: z, pixel count, 2*s (cpu/grid.hpp:106-131).
+__device__ inline void healpix_ring(int n, int i, double& z, int& n_pix, int& s2) {
+  const bool north = i <= 2 * n;
+  const int ii = north ? i : 4 * n - i;
+  double zz;
+  if (ii < n) {
+    zz = 1. - 1. * ii * ii / (3. * n * n);
+    s2 = 2;
+  } else {
+    zz = 4. / 3. - 2. * ii / (3. * n);
+    s2 = 2 * ((ii - n + 1) % 2);
+  }
+  z = north ? zz : -zz;
+  int m = i < 4 * n - i ? i : 4 * n - i;
+  if (m > n) m = n;
+  n_pix = 4 * m;
+}
+
+__device__ inline double healpix_phi(int n, int i, int j) {  // cpu/grid.hpp:107-116
+  const int ii = i <= 2 * n ? i : 4 * n - i;
+  if (ii < n) return kPi / (2. * ii) * (j - 1. / 2.);
+  const double s = (ii - n + 1) % 2;
+  return kPi / (2. * n) * (j - s / 2.);
+}
+
+// Uniform axis with end clamp (cpu/hunt.hpp:155-178).
+__device__ inline int uniform_hunt(double x_min, double x_max, int n, double x, double& a) {
+  if (x <= x_min) {
+    a = 0;
+    return 0;
+  }
+  if (x >= x_max) {
+    a = 1;
+    return n - 2;
+  }
+  const double t = (x - x_min) / (x_max - x_min) * (n - 1);
+  const int i = (int)t;
+  a = t - i;
+  return i;
+}
+
+// Bracket search with VecHunt's result (cpu/hunt.hpp:8-76): lower index of
+// the bracket, clamped to [0, n-2]; linear fraction extrapolates.
+template <typename T>
+__device__ inline int vec_hunt(const T* xx, int n, double x, double& a) {
+  const bool ascnd = xx[n - 1] >= xx[0];
+  int jl = 0, ju = n - 1;
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) >> 1;
+    if ((x >= (double)xx[jm]) == ascnd)
+      jl = jm;
+    else
+      ju = jm;
+  }
+  const int i = clampi(jl, 0, n - 2);
+  a = (x - (double)xx[i]) / ((double)xx[i + 1] - (double)xx[i]);
+  return i;
+}
+
+__device__ inline void lens_ca_lf(const DevLensing& L, double u, double cp, double& ca, double& lf) {
+  double au, ac;  // cpu/lensing.hpp:43-60
+  const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, au);
+  const int ic = uniform_hunt(L.cp_min, L.cp_max, L.n_cp, cp, ac);
+  const double bu = 1. - au, bc = 1. - ac;
+  const double* A = L.ca + (size_t)iu * L.n_cp;
+  const double* F = L.lf + (size_t)iu * L.n_cp;
+  const int n = L.n_cp;
+  ca = bu * bc * A[ic] + au * bc * A[n + ic] + bu * ac * A[ic + 1] + au * ac * A[n + ic + 1];
+  lf = bu * bc * F[ic] + au * bc * F[n + ic] + bu * ac * F[ic + 1] + au * ac * F[n + ic + 1];
+}
+
+__device__ inline double lens_cdt(const DevLensing& L, double u, double ca) {
+  double au, ac;  // cpu/lensing.hpp:62-72
+  const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, au);
+  const int ic = uniform_hunt(L.ca_min, L.ca_max, L.n_ca, ca, ac);
+  const double bu = 1. - au, bc = 1. - ac;
+  const double* D = L.dt + (size_t)iu * L.n_ca;
+  const int n = L.n_ca;
+  return bu * bc * D[ic] + au * bc * D[n + ic] + bu * ac * D[ic + 1] + au * ac * D[n + ic + 1];
+}
+
+// FP64 patch membership (cpu/grid.hpp:144-148 CAP, :158-190 DISKDIFF).
+// Contraction is off so the decisions round like the oracle's.
+__device__ inline bool in_spot(const SpotParams& S, double c2, double s2, double phi) {
+#pragma clang fp contract(off)
+  const double r1 = S.st1 * s2 * cos(phi - S.phi1) + S.ct1 * c2;
+  if (!(r1 > S.cr1)) return false;
+  if (!S.has2) return true;
+  const double r2 = S.st2 * s2 * cos(phi - S.phi2) + S.ct2 * c2;
+  return !(r2 > S.cr2);
+}
+
+__device__ inline int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_min_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_max_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ inline uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ------------------------------------------------------------ ring set-up
+
+constexpr int kMaskWords = 8192 / 64 * 4;  // 4 * 8192 pixels max per ring
+
+__global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, PlainLaunch P) {
+  const int task = blockIdx.x;
+  const RingTask T = P.d_tasks[task];
+  const SpotParams S = P.d_spots[T.spot];
+  RingDesc* D = P.d_desc + task;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nf = P.n_fine, nf1 = nf + 1;
+
+  __shared__ uint64_t s_mask[kMaskWords];
+  __shared__ int s_cnt[kWaves * 2];
+  __shared__ int s_int[kWaves * 2];
+  __shared__ double s_dbl[kWaves * 2];
+  __shared__ int s_base[2];
+  __shared__ int s_nact;
+
+  double z, d_omega;
+  int n_pix, s2;
+  if (T.ring == 0) {  // point-source element (cpu/grid.hpp:17-27)
+    z = S.ps_z;
+    d_omega = S.ps_domega;
+    n_pix = 1;
+    s2 = 0;
+  } else {
+    healpix_ring(S.n_side, T.ring, z, n_pix, s2);
+    d_omega = kPi / (3. * S.n_side * S.n_side);
+  }
+  if (tid == 0) {
+    D->z = z;
+    D->d_omega = d_omega;
+    D->n_pix = n_pix;
+    D->s_shift2 = s2;
+    D->ps_shift = S.ps_shift;
+    D->flags = 0;
+    s_nact = 0;
+    s_base[0] = s_base[1] = 0;
+  }
+  __syncthreads();
+
+  // ---- membership, ballot-compacted into runs of consecutive pixels
+  int n_act = 0, n_runs = 0;
+  if (T.ring == 0) {
+    n_act = 1;
+    n_runs = 0;
+  } else {
+    double c2, sz;
+    if (S.membership == PPM_MEMBER_CAP) {
+      c2 = z;
+      sz = sqrt(1. - z * z);
+    } else {
+      const double th = acos(z);
+      c2 = cos(th);
+      sz = sin(th);
+    }
+    const int n_iter = (n_pix + kBlock - 1) / kBlock;
+    int cnt = 0;
+    for (int it = 0; it < n_iter; ++it) {
+      const int j = 1 + it * kBlock + tid;
+      const bool act = j <= n_pix && in_spot(S, c2, sz, healpix_phi(S.n_side, T.ring, j));
+      const uint64_t m = __ballot(act);
+      if (lane == 0) s_mask[it * kWaves + wave] = m;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) s_cnt[wave] = cnt;
+    __syncthreads();
+    n_act = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    // run starts/ends from neighbouring bits, placed in pixel order
+    for (int it = 0; it < n_iter; ++it) {
+      const int q = it * kBlock + tid;  // 0-based pixel
+      auto bit = [&](int k) -> bool {
+        return k >= 0 && k < n_pix && ((s_mask[k >> 6] >> (k & 63)) & 1ull);
+      };
+      const bool a = bit(q);
+      const bool st = a && !bit(q - 1), en = a && !bit(q + 1);
+      const uint64_t ms = __ballot(st), me = __ballot(en);
+      if (lane == 0) {
+        s_int[wave] = __popcll(ms);
+        s_int[kWaves + wave] = __popcll(me);
+      }
+      __syncthreads();
+      int ps = s_base[0], pe = s_base[1];
+      for (int w = 0; w < wave; ++w) {
+        ps += s_int[w];
+        pe += s_int[kWaves + w];
+      }
+      if (st) {
+        const int k = ps + __popcll(ms & lanemask_lt());
+        if (k < kMaxRuns) D->run_start[k] = q + 1;
+      }
+      if (en) {
+        const int k = pe + __popcll(me & lanemask_lt());
+        if (k < kMaxRuns) D->run_len[k] = q + 1;  // end pixel for now
+      }
+      __syncthreads();
+      if (tid == 0) {
+        for (int w = 0; w < kWaves; ++w) {
+          s_base[0] += s_int[w];
+          s_base[1] += s_int[kWaves + w];
+        }
+      }
+      __syncthreads();
+    }
+    n_runs = s_base[0];
+    __syncthreads();
+    if (tid < kMaxRuns && tid < n_runs) D->run_len[tid] = D->run_len[tid] - D->run_start[tid] + 1;
+    if (tid == 0 && n_runs > kMaxRuns) D->flags |= 1;
+  }
+  if (tid == 0) {
+    D->n_act = n_act;
+    D->n_runs = n_runs < kMaxRuns ? n_runs : kMaxRuns;
+  }
+  if (n_act == 0) {
+    if (tid == 0) D->kept = 0;
+    return;
+  }
+
+  // ---- fine-phase table in FP64 (cpu/os1_cpu.cpp:62-167, cpu/devarshi24st_cpu.cpp:89-212)
+  const double cz = z, sz = sqrt(1. - cz * cz);
+  const double R = S.Re * (1. + S.o2 * cz * cz);
+  const double dRdth = -2. * S.Re * S.o2 * cz * sz;
+  const double u = S.Rs / R, uu = sqrt(1. - u);
+  const double f = dRdth / (R * uu), ff = sqrt(1. + f * f);
+  const double dS = R * R * d_omega * ff;
+  const double cos_tau = 1. / ff, sin_tau = f / ff;
+  const double beta = 2 * kPi * S.nu * R * sz / uu / kCKmS;
+  const double gamma = 1. / sqrt(1. - beta * beta);
+  const double lgR = log1p((S.Re - R) / (R - S.Rs));
+  const double dtR = (S.Re - R + S.Rs * lgR) / kCKmS;
+
+  double* ph = P.d_ph + (size_t)task * nf1;
+  float* fI = P.d_fine + (size_t)task * 3 * nf1;
+  float* zf = fI + nf1;
+  float* cs = zf + nf1;
+
+  int inv_min = nf + 1, inv_max = -1;
+  for (int k = tid; k < nf; k += kBlock) {
+    bool vis = true;
+    double v_fl = 0, v_zf = -1, v_ph = -1, v_cs = 0;
+    do {
+      const double phase = 1. * k / nf;
+      const double sp = 2 * kPi * phase;
+      const double cphi = cos(sp), sphi = sin(sp);
+      double cos_psi, cos_sigma;
+      {
+#pragma clang fp contract(off)
+        cos_psi = S.ci * cz + S.si * sz * cphi;
+      }
+      if (cos_psi < L.cp_min) {
+        vis = false;
+        break;
+      }
+      const double sin_psi = sqrt(1. - cos_psi * cos_psi);
+      double ca, lf;
+      lens_ca_lf(L, u, cos_psi, ca, lf);
+      const double sa = sqrt(1. - ca * ca);
+      const double sa_sp = cos_psi == 1. ? sqrt(lf) : sa / sin_psi;
+      {
+#pragma clang fp contract(off)
+        if (S.variant == PPM_VARIANT_OS1)
+          cos_sigma = ca * cos_tau + sa_sp * sin_tau * (S.ci - cz * cos_psi) / sz;
+        else
+          cos_sigma = ca * cos_tau + sa_sp * sin_tau * (S.ci * sz - S.si * cz * cphi);
+      }
+      if (cos_sigma <= 0) {
+        vis = false;
+        break;
+      }
+      const double cos_xi = -sa_sp * S.si * sphi;
+      const double delta = 1. / (gamma * (1. - beta * cos_xi));
+      const double delta3 = delta * delta * delta;
+      const double dt1 = lens_cdt(L, u, ca) * R / kCKmS;
+      double dt2;
+      if (ca >= 0) {
+        dt2 = dtR;
+      } else {
+        const double tmp = (2. * sa) / sqrt(3. * (1. - u));
+        const double p = -tmp * cos((acos(3. * u / tmp) + 2. * kPi) / 3.) * R;
+        const double lgp = log1p((S.Re - p) / (p - S.Rs));
+        dt2 = 2. * ((S.Re - p + S.Rs * lgp) / kCKmS) - dtR;
+      }
+      const double csp = cos_sigma * delta;
+      double beam = 1.0;
+      if (S.atmosphere == PPM_ATM_BLACKBODY) {
+        if (S.beaming == PPM_BEAM_SIN2)
+          beam = 1. - csp * csp;
+        else if (S.beaming == PPM_BEAM_COS2)
+          beam = csp * csp;
+      }
+      // 1/D^2 and the intensity constants are applied in FP64 at the reduction
+      v_fl = uu * delta3 * csp * lf * (dS * gamma) * beam;
+      v_zf = 1. / (delta * uu);
+      v_ph = phase + (dt1 + dt2) * S.nu;
+      v_cs = csp;
+    } while (0);
+    if (!vis) {
+      inv_min = min(inv_min, k);
+      inv_max = max(inv_max, k);
+    }
+    ph[k] = v_ph;
+    fI[k] = (float)v_fl;
+    zf[k] = (float)v_zf;
+    cs[k] = (float)v_cs;
+  }
+  inv_min = wave_min(inv_min);
+  inv_max = wave_max(inv_max);
+  if (lane == 0) {
+    s_int[wave] = inv_min;
+    s_int[kWaves + wave] = inv_max;
+  }
+  __syncthreads();
+  inv_min = min(min(s_int[0], s_int[1]), min(s_int[2], s_int[3]));
+  inv_max = max(max(s_int[4], s_int[5]), max(s_int[6], s_int[7]));
+  // ring skip (cpu/os1_cpu.cpp:151)
+  if (inv_min == 0 || inv_max == nf - 1) {
+    if (tid == 0) D->kept = 0;
+    return;
+  }
+  if (inv_min <= inv_max) {  // gap fill (cpu/os1_cpu.cpp:152-163)
+    const int beg = inv_min - 1, end = inv_max + 1;
+    const double p0 = ph[beg], p1 = ph[end];
+    const double r0 = zf[beg], r1 = zf[end];
+    __syncthreads();
+    for (int k = inv_min + tid; k <= inv_max; k += kBlock) {
+      ph[k] = p0 + (p1 - p0) * ((double)(k - beg) / (end - beg));
+      zf[k] = (float)(r0 + (r1 - r0) * ((double)(k - beg) / (end - beg)));
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {  // closure (cpu/os1_cpu.cpp:165-167)
+    fI[nf] = fI[0];
+    zf[nf] = zf[0];
+    ph[nf] = 1.0 + ph[0];
+    cs[nf] = cs[0];
+  }
+  __syncthreads();
+  double zmin = 1e300, zmax = -1e300;
+  for (int k = tid; k < nf1; k += kBlock) {
+    zmin = fmin(zmin, (double)zf[k]);
+    zmax = fmax(zmax, (double)zf[k]);
+  }
+  zmin = wave_min_d(zmin);
+  zmax = wave_max_d(zmax);
+  if (lane == 0) {
+    s_dbl[wave] = zmin;
+    s_dbl[kWaves + wave] = zmax;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    D->zf_min = fmin(fmin(s_dbl[0], s_dbl[1]), fmin(s_dbl[2], s_dbl[3]));
+    D->zf_max = fmax(fmax(s_dbl[4], s_dbl[5]), fmax(s_dbl[6], s_dbl[7]));
+    // surface gravity (cpu/devarshi24st_cpu.cpp:63-68, 103-124)
+    double logg = S.logg_override;
+    if (!(logg > 0)) {
+      const double x = S.x;
+      const double ce = -0.791 + 0.776 * x, cp = 1.138 + -1.431 * x;
+      const double de = -1.315 * x + 2.431 * x * x, fe = -1.172 * x;
+      const double dp = 0.653 * x + -2.864 * x * x, fp = 0.975 * x;
+      const double d60 = 13.47 * x + -27.13 * x * x;
+      const double Ob2 = S.Ob * S.Ob, Ob4 = Ob2 * Ob2, Ob6 = Ob4 * Ob2;
+      const double g_g0 = 1. + sz * sz * (ce * Ob2 + de * Ob4 + fe * Ob6) +
+                          cz * cz * (cp * Ob2 + dp * Ob4 + fp * Ob6 - d60 * Ob4) +
+                          fabs(cz) * d60 * Ob4;
+      logg = log10(g_g0) + S.logg0_total;
+    }
+    D->logg = logg;
+    if (N.data) {
+      double aT, ag;
+      D->iT = uniform_hunt(N.logT_min, N.logT_max, N.nT, S.logT, aT);
+      D->ig = uniform_hunt(N.logg_min, N.logg_max, N.ng, logg, ag);
+      D->aT = (float)aT;
+      D->ag = (float)ag;
+    } else {
+      D->iT = D->ig = 0;
+      D->aT = D->ag = 0.f;
+    }
+    D->kept = 1;
+  }
+}
+
+// ---------------------------------------------------------- plain integral
+
+// Per-patch record produced by one lane and read by the whole wavefront.
+struct PatchRec {
+  float c;      // flux/I x GL weight (0: skip)
+  float tx;     // energy-axis offset of this patch, local to the chunk base
+  float zf;     // redshift factor (blackbody)
+  int im;       // first mu node of the stencil
+  float wm[4];  // mu weights
+};
+
+// Byte offsets of the integral kernel's dynamic LDS.
+struct LdsLayout {
+  size_t ph, mu, fI, zf, cs, rec, S, total;
+};
+__host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
+  LdsLayout l;
+  size_t o = 0;
+  l.ph = o;
+  o += (size_t)nf1 * sizeof(double);
+  l.mu = o;
+  o += 80 * sizeof(double);
+  l.fI = o;
+  o += (size_t)nf1 * sizeof(float);
+  l.zf = o;
+  o += (size_t)nf1 * sizeof(float);
+  l.cs = o;
+  o += (size_t)nf1 * sizeof(float);
+  o = (o + 15) & ~(size_t)15;
+  l.rec = o;
+  o += (size_t)kBlock * sizeof(PatchRec);
+  l.S = o;
+  o += (size_t)nsx_rows * kSliceStride * sizeof(float);
+  l.total = (o + 15) & ~(size_t)15;
+  return l;
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int ATM>
+__global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx N, PlainLaunch P) {
+  const int task = blockIdx.x, chunk = blockIdx.y;
+  const RingDesc* Dp = P.d_desc + task;
+  if (!Dp->kept) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nf1 = P.n_fine + 1;
+  const RingTask T = P.d_tasks[task];
+  const SpotParams& S = P.d_spots[T.spot];
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  const LdsLayout lay = integral_layout(nf1, ATM == PPM_ATM_NSX ? N.nE : 0);
+  double* s_ph = reinterpret_cast<double*>(smem + lay.ph);
+  double* s_mu = reinterpret_cast<double*>(smem + lay.mu);
+  float* s_fI = reinterpret_cast<float*>(smem + lay.fI);
+  float* s_zf = reinterpret_cast<float*>(smem + lay.zf);
+  float* s_cs = reinterpret_cast<float*>(smem + lay.cs);
+  PatchRec* s_rec = reinterpret_cast<PatchRec*>(smem + lay.rec);
+  float* s_S = reinterpret_cast<float*>(smem + lay.S);
+
+  // ---- stage the ring's fine tables and the NSX slice
+  {
+    const double* gph = P.d_ph + (size_t)task * nf1;
+    const float* gf = P.d_fine + (size_t)task * 3 * nf1;
+    for (int k = tid; k < nf1; k += kBlock) {
+      s_ph[k] = gph[k];
+      s_fI[k] = gf[k];
+      s_zf[k] = gf[nf1 + k];
+      s_cs[k] = gf[2 * nf1 + k];
+    }
+  }
+  const int n_act = Dp->n_act, n_runs = Dp->n_runs, n_pix = Dp->n_pix, s2 = Dp->s_shift2;
+  const bool point = T.ring == 0;
+  const double ps_shift = Dp->ps_shift;
+  const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
+  if (ATM == PPM_ATM_NSX) {
+    for (int k = tid; k < N.nm; k += kBlock) s_mu[k] = N.mu[k];
+    const int iT = Dp->iT, ig = Dp->ig;
+    const float aT = Dp->aT, ag = Dp->ag;
+    const float w00 = (1.f - aT) * (1.f - ag), w01 = (1.f - aT) * ag, w10 = aT * (1.f - ag),
+                w11 = aT * ag;
+    const size_t sT = (size_t)N.ng * N.nE * N.nm, sg = (size_t)N.nE * N.nm;
+    const float* c00 = N.data + iT * sT + ig * sg;
+    const float* c01 = c00 + sg;
+    const float* c10 = c00 + sT;
+    const float* c11 = c10 + sg;
+    const int total = N.nE * kSliceStride;
+    for (int k = tid; k < total; k += kBlock) {
+      const int r = k / kSliceStride, m = k - r * kSliceStride;
+      float v = 0.f;
+      if (m < N.nm) {
+        const size_t o = (size_t)r * N.nm + m;
+        v = w00 * c00[o] + w01 * c01[o] + w10 * c10[o] + w11 * c11[o];
+      }
+      s_S[k] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- per-lane energy
+  const int e = chunk * kEChunk + lane;
+  const bool e_ok = e < P.n_e;
+  const double E = P.d_energies[e_ok ? e : P.n_e - 1];
+  // energy axis position t = (log10(E zf / kT) - logE_min) / h is split into
+  // a lane part (le) and a patch part (tx) around an integer base r0 so both
+  // stay small in FP32
+  const double E_mid = P.d_energies[min(chunk * kEChunk + 32, P.n_e - 1)];
+  const double zf_mid = 0.5 * (Dp->zf_min + Dp->zf_max);
+  const double t_mid = (log10(E_mid * zf_mid / S.kT_nsx) - N.logE_min) / h;
+  const int r0 = ATM == PPM_ATM_NSX ? clampi((int)floor(t_mid), 0, N.nE - 1) : 0;
+  const float le = (float)((log10(E) - log10(E_mid)) / h);
+  const double tx_base = (log10(E_mid) - log10(S.kT_nsx) - N.logE_min) / h - r0;
+  const float t_lo = (float)(-r0), t_hi = (float)(N.nE - 1 - r0);
+  const int st_lo = -r0, st_hi = N.nE - 4 - r0;
+  const float Ef = (float)E;
+  const float inv_kT = (float)S.inv_kT;
+  const float* s_Sr0 = s_S + r0 * kSliceStride;
+
+  PatchRec* rec = s_rec + wave * 64;
+  const int ng = P.n_pts / P.n_phase;
+
+  for (int b = wave; b < P.n_phase; b += kWaves) {
+    float acc = 0.f, comp = 0.f;
+    for (int g = 0; g < ng; ++g) {
+      const PhasePoint pt = P.d_pts[b * ng + g];
+      for (int base = 0; base < n_act; base += 64) {
+        // ---- patch set-up, one patch per lane, FP64
+        const int k = base + lane;
+        PatchRec r;
+        r.c = 0.f;
+        r.tx = 0.f;
+        r.zf = 0.f;
+        r.im = 0;
+        r.wm[0] = r.wm[1] = r.wm[2] = r.wm[3] = 0.f;
+        if (k < n_act) {
+          double shift;
+          if (point) {
+            shift = ps_shift;
+          } else {
+            int kk = k, j = 0;
+            for (int q = 0; q < n_runs; ++q) {
+              const int len = Dp->run_len[q];
+              if (kk < len) {
+                j = Dp->run_start[q] + kk;
+                break;
+              }
+              kk -= len;
+            }
+            shift = (j - 0.5 * s2 * 0.5) / n_pix;
+          }
+          // time-delayed phase (cpu/devarshi24st_cpu.cpp:218-224)
+          const double v = pt.p + shift + 1.;
+          double pp = v - floor(v);
+          if (pp < s_ph[0]) pp += 1.;
+          double a;
+          const int i = vec_hunt(s_ph, nf1, pp, a);
+          const double fI = (1 - a) * (double)s_fI[i] + a * (double)s_fI[i + 1];
+          if (fI != 0) {
+            const double zz = (1 - a) * (double)s_zf[i] + a * (double)s_zf[i + 1];
+            r.c = (float)(fI * pt.w);
+            r.zf = (float)zz;
+            if (ATM == PPM_ATM_NSX) {
+              double mu = (1 - a) * (double)s_cs[i] + a * (double)s_cs[i + 1];
+              if (mu < N.mu_min) mu = N.mu_min;
+              double am;
+              int im = vec_hunt(s_mu, N.nm, mu, am);
+              if (im >= N.nm - 5) {  // linear at the edge (cpu/nsx.hpp:272-276)
+                r.wm[0] = (float)(1 - am);
+                r.wm[1] = (float)am;
+              } else {  // cubic (cpu/nsx.hpp:277-281)
+                im = clampi(im - 1, 0, N.nm - 4);
+                const double x0 = s_mu[im], x1 = s_mu[im + 1], x2 = s_mu[im + 2], x3 = s_mu[im + 3];
+                r.wm[0] = (float)((mu - x1) * (mu - x2) * (mu - x3) / ((x0 - x1) * (x0 - x2) * (x0 - x3)));
+                r.wm[1] = (float)((mu - x0) * (mu - x2) * (mu - x3) / ((x1 - x0) * (x1 - x2) * (x1 - x3)));
+                r.wm[2] = (float)((mu - x0) * (mu - x1) * (mu - x3) / ((x2 - x0) * (x2 - x1) * (x2 - x3)));
+                r.wm[3] = (float)((mu - x0) * (mu - x1) * (mu - x2) / ((x3 - x0) * (x3 - x1) * (x3 - x2)));
+              }
+              r.im = im;
+              r.tx = (float)(tx_base + log10(zz) / h);
+            }
+          }
+        }
+        rec[lane] = r;
+        wave_sync();
+        // ---- evaluation, one energy per lane
+        const int nk = min(64, n_act - base);
+        float bsum = 0.f;
+        for (int q = 0; q < nk; ++q) {
+          const PatchRec& R = rec[q];
+          if (R.c == 0.f) continue;
+          float I;
+          if (ATM == PPM_ATM_NSX) {
+            float t = le + R.tx;
+            t = fminf(fmaxf(t, t_lo), t_hi);
+            const int st = clampi((int)floorf(t) - 1, st_lo, st_hi);
+            const float vv = t - (float)st;
+            const float a0 = vv, a1 = vv - 1.f, a2 = vv - 2.f, a3 = vv - 3.f;
+            const float w0 = -(a1 * a2 * a3) * (1.f / 6.f);
+            const float w1 = (a0 * a2 * a3) * 0.5f;
+            const float w2 = -(a0 * a1 * a3) * 0.5f;
+            const float w3 = (a0 * a1 * a2) * (1.f / 6.f);
+            const float* row = s_Sr0 + st * kSliceStride + R.im;
+            float rr[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+              const float* rw = row + kk * kSliceStride;
+              rr[kk] = R.wm[0] * rw[0] + R.wm[1] * rw[1] + R.wm[2] * rw[2] + R.wm[3] * rw[3];
+            }
+            I = w0 * rr[0] + w1 * rr[1] + w2 * rr[2] + w3 * rr[3];
+          } else {
+            const float Ee = Ef * R.zf;
+            I = Ee * Ee * Ee / expm1f(Ee * inv_kT);
+          }
+          bsum += R.c * I;
+        }
+        wave_sync();
+        // Kahan-compensated per-bin accumulation
+        const float y = bsum - comp;
+        const float tt = acc + y;
+        comp = (tt - acc) - y;
+        acc = tt;
+      }
+    }
+    if (e_ok) P.d_partial[((size_t)task * P.n_phase + b) * P.n_e + e] = acc - comp;
+  }
+}
+
+// ------------------------------------------------------------------ reduce
+
+__global__ void k_reduce(PlainLaunch P) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_out = P.n_phase * P.n_e;
+  if (idx >= n_out) return;
+  const int e = idx % P.n_e;
+  double total = 0.0, part = 0.0;
+  int spot = -1;
+  double scale = 0.0;
+  for (int t = 0; t < P.n_tasks; ++t) {
+    const int s = P.d_tasks[t].spot;
+    if (s != spot) {
+      total += part * scale;
+      part = 0.0;
+      spot = s;
+      scale = P.d_spots[s].scale;
+    }
+    if (P.d_desc[t].kept) part += (double)P.d_partial[(size_t)t * n_out + idx];
+  }
+  total += part * scale;
+  P.d_flux[idx] = total / P.d_energies[e];
+}
+
+// ------------------------------------------------------------------- fold
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// counts[b][c][p] = scale[b] * sum_e rsp[c][e] * flux[b][p][e]; one wave per
+// 16 (channel) x 16 (phase) tile, K in steps of 16 energies.
+// v_mfma_f32_16x16x16bf16_1k lane layout: A[m = l%16][k = 4(l/16)+i],
+// B[k = 4(l/16)+i][n = l%16], D[m = 4(l/16)+i][n = l%16].
+__global__ __launch_bounds__(64) void k_fold(const uint16_t* __restrict__ rsp_hi,
+                                             const uint16_t* __restrict__ rsp_lo, int n_ch,
+                                             int n_e, const double* __restrict__ flux, int n_phase,
+                                             const double* __restrict__ scale,
+                                             double* __restrict__ counts) {
+  const int c0 = blockIdx.x * 16, p0 = blockIdx.y * 16, b = blockIdx.z;
+  const int l = threadIdx.x, row = l & 15, kq = (l >> 4) * 4;
+  const int ch = c0 + row, ph = p0 + row;
+  const double* fb = flux + (size_t)b * n_phase * n_e;
+  float4_t acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < n_e; k0 += 16) {
+    short4_t ah, al, bh, bl;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = k0 + kq + i;
+      uint16_t h = 0, lo = 0;
+      if (ch < n_ch && e < n_e) {
+        h = rsp_hi[(size_t)ch * n_e + e];
+        lo = rsp_lo[(size_t)ch * n_e + e];
+      }
+      ah[i] = (short)h;
+      al[i] = (short)lo;
+      float fv = 0.f;
+      if (ph < n_phase && e < n_e) fv = (float)fb[(size_t)ph * n_e + e];
+      const uint16_t fh = bf16_rne(fv);
+      const uint16_t fl = bf16_rne(fv - bf16_to_f(fh));
+      bh[i] = (short)fh;
+      bl[i] = (short)fl;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+  }
+  const double sc = scale[b];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int c = c0 + kq + i, p = p0 + row;
+    if (c < n_ch && p < n_phase) counts[((size_t)b * n_ch + c) * n_phase + p] = sc * (double)acc[i];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+size_t integral_lds_bytes(int n_fine, int atmosphere, int nsx_rows) {
+  return integral_layout(n_fine + 1, atmosphere == PPM_ATM_NSX ? nsx_rows : 0).total;
+}
+
+int max_dynamic_lds(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return 65536;
+  return v;
+}
+
+int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream) {
+  if (P.n_tasks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ring_setup, dim3(P.n_tasks), dim3(kBlock), 0, (hipStream_t)stream, L, N, P);
+  return hipGetLastError();
+}
+
+int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream) {
+  if (P.n_tasks == 0) return hipSuccess;
+  const size_t lds = integral_lds_bytes(P.n_fine, P.atmosphere, N.nE);
+  dim3 grid(P.n_tasks, P.n_echunks);
+  if (P.atmosphere == PPM_ATM_NSX) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<PPM_ATM_NSX>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_integral_plain<PPM_ATM_NSX>, grid, dim3(kBlock), lds, (hipStream_t)stream, L, N, P);
+  } else {
+    hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<PPM_ATM_BLACKBODY>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_integral_plain<PPM_ATM_BLACKBODY>, grid, dim3(kBlock), lds, (hipStream_t)stream, L, N, P);
+  }
+  return hipGetLastError();
+}
+
+int launch_reduce(const PlainLaunch& P, const int*, void* stream) {
+  const int n = P.n_phase * P.n_e;
+  hipLaunchKernelGGL(k_reduce, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, P);
+  return hipGetLastError();
+}
+
+int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_e,
+                const double* d_flux, int n_batch, int n_phase, const double* d_scale,
+                double* d_counts, void* stream) {
+  dim3 grid((n_ch + 15) / 16, (n_phase + 15) / 16, n_batch);
+  hipLaunchKernelGGL(k_fold, grid, dim3(64), 0, (hipStream_t)stream, rsp_hi, rsp_lo, n_ch, n_e,
+                     d_flux, n_phase, d_scale, d_counts);
+  return hipGetLastError();
+}
+
+}  // namespace ppm

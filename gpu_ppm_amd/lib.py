@@ -1,0 +1,212 @@
+"""Python host API over libppm's C ABI.
+
+Mirrors the reference's host interface for this path: the table objects
+(Lens, NSX, IsmInst: gpu/lensing.cuh, gpu/nsx.cuh, gpu/ism_inst.cuh) become
+one ``Context`` that loads them into HBM once, and ``calc_one_star``
+(gpu/devarshi24st_gpu.cu:431-511, gpu/os1_gpu.cu:303-349) becomes
+``Context.eval`` (flux and counts for one or many stars).  Errors raise
+``PpmError`` carrying the library's status code instead of exiting.
+"""
+import ctypes
+import os
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+from . import abi
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libppm.so")
+
+
+class PpmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("ppm status %d: %s" % (status, message))
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load libppm.so from the package directory (raises if it is missing:
+    there is no fallback implementation)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libppm.so is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ppm_abi_version.restype = c_int
+    L.ppm_strerror.restype = c_char_p
+    L.ppm_strerror.argtypes = [c_int]
+    L.ppm_last_error.restype = c_char_p
+    L.ppm_last_error.argtypes = [c_void_p]
+    L.ppm_open.restype = c_int
+    L.ppm_open.argtypes = [POINTER(abi.OpenOpts), POINTER(c_void_p)]
+    L.ppm_close.restype = c_int
+    L.ppm_close.argtypes = [c_void_p]
+    L.ppm_response_info.restype = c_int
+    L.ppm_response_info.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_double)]
+    L.ppm_lensing_domain.restype = c_int
+    L.ppm_lensing_domain.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+    L.ppm_eval.restype = c_int
+    L.ppm_eval.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), POINTER(c_double),
+                           POINTER(c_double), POINTER(abi.Stats)]
+    L.ppm_eval_device.restype = c_int
+    L.ppm_eval_device.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), c_void_p, c_void_p,
+                                  POINTER(abi.Stats)]
+    L.ppm_count_active.restype = c_int
+    L.ppm_count_active.argtypes = [c_void_p, POINTER(abi.Star), POINTER(abi.Grid), POINTER(c_int64)]
+    L.ppm_fold_device.restype = c_int
+    L.ppm_fold_device.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, POINTER(c_double), c_void_p]
+    L.ppm_submit.restype = c_int
+    L.ppm_submit.argtypes = [c_void_p, POINTER(abi.Star), POINTER(abi.Grid), POINTER(c_uint64)]
+    L.ppm_poll.restype = c_int
+    L.ppm_poll.argtypes = [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+    L.ppm_cancel.restype = c_int
+    L.ppm_cancel.argtypes = [c_void_p, c_uint64]
+    L.ppm_drain.restype = c_int
+    L.ppm_drain.argtypes = [c_void_p]
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _star_array(stars):
+    if isinstance(stars, abi.Star):
+        stars = [stars]
+    arr = (abi.Star * len(stars))()
+    for k, s in enumerate(stars):
+        arr[k] = s
+    return arr
+
+
+class Context:
+    """Tables resident in HBM plus the evaluation entry points."""
+
+    def __init__(self, root=".", lensing="std", nsx_path=None, responses=(), device=0,
+                 nsx_config=None, hbm_budget_bytes=0):
+        L = load()
+        self._keep = []
+        opts = abi.OpenOpts()
+        opts.root = os.fsencode(root)
+        opts.lensing_setting = lensing.encode()
+        # NULL -> default path under root; "" -> no NSX grid
+        opts.nsx_path = None if nsx_path is None else os.fsencode(nsx_path)
+        opts.response_settings = ",".join(responses).encode()
+        if nsx_config is not None:
+            self._keep.append(nsx_config)
+            opts.nsx_config = ctypes.pointer(nsx_config)
+        opts.device = device
+        opts.hbm_budget_bytes = hbm_budget_bytes
+        h = c_void_p()
+        st = L.ppm_open(byref(opts), byref(h))
+        if st != abi.OK:
+            raise PpmError(st, L.ppm_strerror(st).decode())
+        self._h = h
+        self.responses = list(responses)
+
+    # -- helpers
+    def _check(self, st):
+        if st != abi.OK:
+            msg = load().ppm_last_error(self._h).decode() or load().ppm_strerror(st).decode()
+            raise PpmError(st, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ppm_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def response_info(self, k):
+        n_ch, n_e = c_int(), c_int()
+        self._check(load().ppm_response_info(self._h, k, byref(n_ch), byref(n_e), None))
+        e = np.zeros(n_e.value)
+        self._check(load().ppm_response_info(self._h, k, None, None, _dptr(e)))
+        return n_ch.value, e
+
+    def lensing_domain(self):
+        a, b, c = c_double(), c_double(), c_double()
+        self._check(load().ppm_lensing_domain(self._h, byref(a), byref(b), byref(c)))
+        return a.value, b.value, c.value
+
+    def n_energy(self, grid):
+        if grid.energies_keV and grid.n_energy > 0:
+            return grid.n_energy
+        return self.response_info(grid.response)[1].size
+
+    # -- evaluation
+    def eval(self, stars, grid, counts=True):
+        """Flux [n][n_phase][n_energy] and counts [n][n_ch][n_phase] (None
+        without a response) plus a stats struct."""
+        arr = _star_array(stars)
+        n = len(arr)
+        ne = self.n_energy(grid)
+        flux = np.zeros((n, grid.n_phase, ne))
+        cnt = None
+        if counts and grid.response >= 0:
+            n_ch = self.response_info(grid.response)[0]
+            cnt = np.zeros((n, n_ch, grid.n_phase))
+        stats = abi.Stats()
+        self._check(load().ppm_eval(self._h, arr, n, byref(grid), _dptr(flux),
+                                    _dptr(cnt) if cnt is not None else None, byref(stats)))
+        return flux, cnt, stats
+
+    def eval_device(self, stars, grid, d_flux, d_counts=None):
+        """Outputs to device pointers (ints, e.g. torch tensor data_ptr())."""
+        arr = _star_array(stars)
+        stats = abi.Stats()
+        self._check(load().ppm_eval_device(self._h, arr, len(arr), byref(grid), c_void_p(d_flux),
+                                           c_void_p(d_counts) if d_counts else None, byref(stats)))
+        return stats
+
+    def count_active(self, star, grid):
+        n = c_int64()
+        self._check(load().ppm_count_active(self._h, byref(star), byref(grid), byref(n)))
+        return n.value
+
+    def fold_device(self, response, n_batch, n_phase, d_flux, scale, d_counts):
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        self._check(load().ppm_fold_device(self._h, response, n_batch, n_phase, c_void_p(d_flux),
+                                           _dptr(sc), c_void_p(d_counts)))
+
+    # -- serving
+    def submit(self, star, grid):
+        t = c_uint64()
+        self._check(load().ppm_submit(self._h, byref(star), byref(grid), byref(t)))
+        return t.value
+
+    def poll(self, ticket, grid):
+        """(status, flux, counts, latency_ms); status PENDING until done."""
+        ne = self.n_energy(grid)
+        flux = np.zeros((grid.n_phase, ne))
+        cnt = None
+        if grid.response >= 0:
+            cnt = np.zeros((self.response_info(grid.response)[0], grid.n_phase))
+        lat = c_double()
+        st = load().ppm_poll(self._h, ticket, _dptr(flux), _dptr(cnt) if cnt is not None else None,
+                             byref(lat))
+        if st not in (abi.OK, abi.PENDING):
+            self._check(st)
+        return st, flux, cnt, lat.value
+
+    def cancel(self, ticket):
+        return load().ppm_cancel(self._h, ticket)
+
+    def drain(self):
+        self._check(load().ppm_drain(self._h))

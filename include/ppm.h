@@ -1,0 +1,209 @@
+/*
+ * ppm.h - C ABI of libppm, an MI355X (gfx950) implementation of the
+ * pulse-profile hot path of fizban007/gpu_ppm.
+ *
+ * The reference exposes no library: its boundary is a process that reads
+ * table files and writes output files.  The entry points below are the ones
+ * a foreign-function binding of that path would need; each names the
+ * reference routine it replaces.  Conventions:
+ *   - every call returns a ppm_status; nothing exits or throws across the ABI;
+ *   - host buffers are owned by the caller; an opaque ppm_ctx owns all device
+ *     memory (tables stay resident in HBM until ppm_close);
+ *   - one ctx is used from one host thread at a time, except ppm_submit /
+ *     ppm_poll / ppm_cancel, which are internally synchronised;
+ *   - plain C types only (no HIP or torch types).
+ */
+#ifndef PPM_H
+#define PPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPM_ABI_VERSION 1
+#define PPM_MAX_SPOTS 4
+
+typedef enum ppm_status {
+  PPM_OK = 0,
+  PPM_ERR_ARG = 1,        /* invalid argument or parameter out of domain   */
+  PPM_ERR_IO = 2,         /* table file missing or unreadable              */
+  PPM_ERR_FORMAT = 3,     /* table contents malformed or inconsistent      */
+  PPM_ERR_DEVICE = 4,     /* HIP runtime error or no usable device         */
+  PPM_ERR_NOMEM = 5,      /* request exceeds the device-memory budget      */
+  PPM_ERR_BUSY = 6,       /* serving queue full                            */
+  PPM_ERR_NOT_FOUND = 7,  /* unknown ticket                                */
+  PPM_ERR_CANCELLED = 8,  /* request was cancelled                         */
+  PPM_PENDING = 9,        /* request admitted but not finished             */
+  PPM_ERR_UNSUPPORTED = 10 /* feature not available in this build / ctx    */
+} ppm_status;
+
+/* Atmosphere model of a spot. */
+enum {
+  PPM_ATM_BLACKBODY = 0, /* Ibb(E, kT), cpu/unit.hpp:28-31, os1 path            */
+  PPM_ATM_NSX = 1        /* NSX grid, 2l2c_le interpolation, cpu/nsx.hpp:248-296 */
+};
+
+/* Beaming of the blackbody atmosphere (cpu/os1_cpu.cpp:128-136). */
+enum { PPM_BEAM_ISOTROPIC = 'i', PPM_BEAM_COS2 = 'c', PPM_BEAM_SIN2 = 's' };
+
+/* Patch-membership rule of a spot.  Both are FP64 tests on HEALPix pixel
+ * centres; they differ in arithmetic only:
+ *   CAP:      cos rho = sin th1 sqrt(1-z^2) cos(phi-phi1) + cos th1 z  (cpu/grid.hpp:94-156)
+ *   DISKDIFF: theta = acos z, disk1 \ disk2 via sin/cos of theta       (cpu/grid.hpp:158-250) */
+enum { PPM_MEMBER_CAP = 0, PPM_MEMBER_DISKDIFF = 1 };
+
+/* Form of the cos(sigma) expression of the per-element kernel.  Analytically
+ * equal; kept to reproduce each reference program to the last bits.
+ *   OS1:      cpu/os1_cpu.cpp:95-96
+ *   DEVARSHI: cpu/devarshi24st_cpu.cpp:145-147 */
+enum { PPM_VARIANT_OS1 = 0, PPM_VARIANT_DEVARSHI = 1 };
+
+/* Phase sampling: os1 samples points k/n_phase (cpu/os1_cpu.cpp:36,169-181);
+ * devarshi integrates each bin with n_gl Gauss-Legendre nodes
+ * (cpu/devarshi24st_cpu.cpp:214-239). */
+enum { PPM_SAMPLE_POINTS = 0, PPM_SAMPLE_GL = 1 };
+
+/* Evaluation method. */
+enum {
+  PPM_METHOD_AUTO = 0,    /* PLAIN for one star, BATCHED for several           */
+  PPM_METHOD_PLAIN = 1,   /* reference iteration space, one profile per launch */
+  PPM_METHOD_BATCHED = 2  /* persistent work queue + phase-shift dedupe        */
+};
+
+typedef struct ppm_spot {
+  double theta1, phi1, angrad1; /* disk 1: centre colatitude, azimuth, radius (rad) */
+  double theta2, phi2, angrad2; /* disk 2 (subtracted) when angrad2 > 0              */
+  double kT_keV;                /* temperature; NSX uses logT = log10(kT / k_B)      */
+  int membership;               /* PPM_MEMBER_*                                      */
+  int beaming;                  /* PPM_BEAM_* (blackbody only)                       */
+} ppm_spot;
+
+typedef struct ppm_star {
+  double mass_msun;        /* M                                         */
+  double radius_eq_km;     /* equatorial radius R_eq                    */
+  double spin_hz;          /* rotation frequency nu                     */
+  double inclination_rad;  /* observer inclination i                    */
+  double distance_km;      /* D                                         */
+  double exposure_s;       /* fold scale (cpu/devarshi24st_cpu.cpp:255) */
+  double logg_override;    /* > 0: use this log g for every ring        */
+  double point_source_below; /* os1 shortcut: a spot with angrad1 below
+                                this is one element (cpu/os1_cpu.cpp:40-42) */
+  int atmosphere;          /* PPM_ATM_*                                 */
+  int variant;             /* PPM_VARIANT_*                             */
+  int n_spots;             /* 1..PPM_MAX_SPOTS; contributions add       */
+  int reserved;
+  ppm_spot spot[PPM_MAX_SPOTS];
+} ppm_star;
+
+typedef struct ppm_grid {
+  int n_side;        /* HEALPix N_side                                      */
+  int n_phase;       /* output phase points or bins                         */
+  int sampling;      /* PPM_SAMPLE_*                                        */
+  int n_gl;          /* GL nodes per bin (1..6), PPM_SAMPLE_GL only         */
+  int n_fine;        /* fine rotational-phase nodes per ring                */
+  int n_energy;      /* number of observed energies                         */
+  const double* energies_keV; /* [n_energy]; NULL -> the response's E_obs   */
+  int response;      /* index into the ctx's response list, -1: no fold      */
+  int method;        /* PPM_METHOD_*                                         */
+} ppm_grid;
+
+/* Dimensions and axes of an NSX grid (cpu/nsx.hpp:11-53). */
+typedef struct ppm_nsx_config {
+  int n_logT, n_logg, n_logEkT, n_mu;
+  double logT_min, logT_max;
+  double logg_min, logg_max;
+  double logEkT_min, logEkT_max;
+} ppm_nsx_config;
+
+typedef struct ppm_open_opts {
+  const char* root;              /* directory holding tables/ and ext/ ("." if NULL) */
+  const char* lensing_setting;   /* tables/lensing/<setting>; NULL -> "std"          */
+  const char* nsx_path;          /* NULL -> ext/model_data/nsx_H_v200804.out;
+                                    "" -> no NSX grid                                */
+  const char* response_settings; /* comma list, tables/ism_inst/<s>; NULL/"" -> none */
+  const ppm_nsx_config* nsx_config; /* NULL -> nsx_H_v200804 (35x14x166x67)        */
+  int device;                    /* HIP device ordinal                               */
+  int reserved;
+  size_t hbm_budget_bytes;       /* serving admission budget; 0 -> 16 GiB            */
+} ppm_open_opts;
+
+typedef struct ppm_stats {
+  double kernel_ms;        /* set-up + integral + reduction (hipEvent)       */
+  double integral_ms;      /* the integral kernel alone (hipEvent)           */
+  double fold_ms;          /* response fold (hipEvent)                       */
+  double total_ms;         /* event pair around integral + fold              */
+  int64_t n_active;        /* active patches summed over stars and spots     */
+  int64_t n_eval_ref;      /* evaluations in the reference iteration space  */
+  int64_t n_eval_done;     /* (phase point, energy) evaluations performed    */
+  int64_t n_rings;         /* ring tasks with active patches                 */
+} ppm_stats;
+
+typedef struct ppm_ctx ppm_ctx;
+
+/* Version / errors. */
+int ppm_abi_version(void);
+const char* ppm_strerror(ppm_status status);
+/* Message of the last failed call on ctx (thread-unsafe view; "" if none). */
+const char* ppm_last_error(const ppm_ctx* ctx);
+
+/* Load the reference-format tables and keep them resident in HBM.
+ * Replaces Lens::Lens (gpu/lensing.cuh:27-54), NSX::NSX (gpu/nsx.cuh:83-128)
+ * and IsmInst::IsmInst (gpu/ism_inst.cuh:18-32). */
+ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out_ctx);
+
+/* Release all device memory and the serving thread.  Replaces
+ * free_gpu_memory() (gpu/lensing.cuh:56-61, gpu/nsx.cuh:130-134,
+ * gpu/ism_inst.cuh:34-38). */
+ppm_status ppm_close(ppm_ctx* ctx);
+
+/* Table facts needed by callers (response sizes, lensing domain). */
+ppm_status ppm_response_info(const ppm_ctx* ctx, int response, int* n_ch, int* n_energy,
+                             double* energies_keV /* [n_energy] or NULL */);
+ppm_status ppm_lensing_domain(const ppm_ctx* ctx, double* u_min, double* u_max,
+                              double* cos_psi_min);
+
+/* Evaluate n profiles on one grid.  flux is [n][n_phase][n_energy] (FP64);
+ * counts, if non-NULL and grid->response >= 0, is [n][n_ch][n_phase].
+ * Host buffers; blocking.  Replaces calc_one_star
+ * (gpu/os1_gpu.cu:303-349, gpu/devarshi24st_gpu.cu:431-511). */
+ppm_status ppm_eval(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid,
+                    double* flux, double* counts, ppm_stats* stats);
+
+/* Same as ppm_eval with device-resident outputs (HBM pointers) and no host
+ * copies; returns after the work is complete.  This is the form the
+ * benchmark times (inputs and outputs stay in HBM). */
+ppm_status ppm_eval_device(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid,
+                           double* d_flux, double* d_counts, ppm_stats* stats);
+
+/* Number of active patches of one star on a grid (FP64 membership on the
+ * device).  Replaces the N_active print of GenerateHealpixGrid
+ * (cpu/grid.hpp:247). */
+ppm_status ppm_count_active(ppm_ctx* ctx, const ppm_star* star, const ppm_grid* grid,
+                            int64_t* n_active);
+
+/* Response fold on the device: counts[b][c][p] = scale[b] * sum_e
+ * rsp[c][e] * flux[b][p][e] using split-BF16 MFMA with FP32 accumulation.
+ * d_flux / d_counts are device pointers.  Replaces MatMul::operator()
+ * (gpu/matmul.cuh:11-35). */
+ppm_status ppm_fold_device(ppm_ctx* ctx, int response, int n_batch, int n_phase,
+                           const double* d_flux, const double* scale, double* d_counts);
+
+/* Serving: continuous batching of single-profile requests.  Requests are
+ * admitted against the HBM budget, packed into persistent batched launches
+ * while others run, and completed out of order.  The grid's energy array is
+ * copied at submit.  ppm_poll returns PPM_PENDING until the result is ready,
+ * then copies flux [n_phase][n_energy] and counts [n_ch][n_phase] (if asked
+ * for) and releases the ticket.  latency_ms (optional) receives submit->done. */
+ppm_status ppm_submit(ppm_ctx* ctx, const ppm_star* star, const ppm_grid* grid, uint64_t* ticket);
+ppm_status ppm_poll(ppm_ctx* ctx, uint64_t ticket, double* flux, double* counts, double* latency_ms);
+ppm_status ppm_cancel(ppm_ctx* ctx, uint64_t ticket);
+/* Block until every admitted request is finished or cancelled. */
+ppm_status ppm_drain(ppm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPM_H */
