@@ -72,25 +72,24 @@ def cpu_baseline(root, star, grid, target_s=15.0):
 
 
 def bench_plain(root, steps, warmup):
-    import torch
-    from gpu_ppm_amd import abi
+    from gpu_ppm_amd import abi, hip
     from gpu_ppm_amd import scenarios as S
     from gpu_ppm_amd.lib import Context
     star = S.cfg1_star()
     grid = abi.make_grid(1024, 64, sampling=abi.SAMPLE_GL, n_gl=2, response=0, method=abi.METHOD_PLAIN)
     ctx = Context(root, lensing="std", responses=("std",))
     n_ch, e = ctx.response_info(0)
-    d_flux = torch.zeros((1, 64, e.size), dtype=torch.float64, device="cuda")
-    d_cnt = torch.zeros((1, n_ch, 64), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    d_flux = hip.DeviceArray((1, 64, e.size))
+    d_cnt = hip.DeviceArray((1, n_ch, 64))
+    hip.synchronize()
     for _ in range(warmup):
-        ctx.eval_device(star, grid, d_flux.data_ptr(), d_cnt.data_ptr())
-    torch.cuda.synchronize()
+        ctx.eval_device(star, grid, d_flux.ptr, d_cnt.ptr)
+    hip.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        stats.append(ctx.eval_device(star, grid, d_flux.data_ptr(), d_cnt.data_ptr()))
-    torch.cuda.synchronize()
+        stats.append(ctx.eval_device(star, grid, d_flux.ptr, d_cnt.ptr))
+    hip.synchronize()
     dt = time.perf_counter() - t0
     integral_ms = sum(s.integral_ms for s in stats) / steps
     n_eval = stats[0].n_eval_done

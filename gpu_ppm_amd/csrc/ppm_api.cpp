@@ -268,9 +268,34 @@ struct Upload {
 
 // One star, plain path: the reference iteration space with one launch
 // sequence per profile.  Outputs are device pointers.
+// NSX slice rows one workgroup stages: the widest energy chunk plus room for
+// the redshift spread of a ring (a ring that needs more is flagged on the
+// device and the profile is recomputed with the whole slice).
+int nsx_rows_for(const ppm_ctx* c, const std::vector<double>& energies) {
+  const int per = energies_per_chunk();
+  const double h = (c->N.logE_max - c->N.logE_min) / (c->N.nE - 1);
+  double span = 0;
+  for (size_t e0 = 0; e0 < energies.size(); e0 += per) {
+    const size_t e1 = std::min(energies.size(), e0 + per) - 1;
+    span = std::max(span, std::fabs(std::log10(energies[e1] / energies[e0])) / h);
+  }
+  const int rows = (int)std::ceil(span) + 32 + 8;
+  return std::min(rows, c->N.nE);
+}
+
+ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
+                           const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
+                           double* d_scale_slot, ppm_stats* stats, int rows_override);
+
 ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                       const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
                       double* d_scale_slot, ppm_stats* stats) {
+  return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, 0);
+}
+
+ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
+                           const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
+                           double* d_scale_slot, ppm_stats* stats, int rows_override) {
   const int n_e = (int)energies.size();
   const int nf1 = g.n_fine + 1;
   std::vector<SpotParams> sp(st.n_spots);
@@ -282,8 +307,8 @@ ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const s
     for (int r : rings) tasks.push_back(RingTask{s, r});
   }
   const int n_tasks = (int)tasks.size();
-  if (st.atmosphere == PPM_ATM_NSX &&
-      integral_lds_bytes(g.n_fine, st.atmosphere, c->N.nE) > (size_t)c->max_lds)
+  int nsx_rows = st.atmosphere == PPM_ATM_NSX ? (rows_override ? rows_override : nsx_rows_for(c, energies)) : 0;
+  if (integral_lds_bytes(g.n_fine, st.atmosphere, nsx_rows) > (size_t)c->max_lds)
     return fail(c, PPM_ERR_ARG, "n_fine too large for the LDS of one workgroup");
   Workspace& W = c->ws;
   int e = 0;
@@ -293,7 +318,8 @@ ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const s
       (e = W.ph.ensure(sizeof(double) * nf1 * std::max(n_tasks, 1))) ||
       (e = W.fine.ensure(sizeof(float) * 3 * nf1 * std::max(n_tasks, 1))) ||
       (e = W.pts.ensure(sizeof(PhasePoint) * pts.size())) || (e = W.energies.ensure(sizeof(double) * n_e)) ||
-      (e = W.partial.ensure(sizeof(float) * n_part)))
+      (e = W.partial.ensure(sizeof(float) * n_part)) ||
+      (e = W.red.ensure(sizeof(double) * reduce_groups(n_tasks) * g.n_phase * n_e)))
     return hip_fail(c, e, "device allocation");
   Upload up;
   up.add(W.spots.p, sp.data(), sizeof(SpotParams) * sp.size());
@@ -317,8 +343,9 @@ ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const s
   P.n_pts = (int)pts.size();
   P.n_phase = g.n_phase;
   P.n_e = n_e;
-  P.n_echunks = (n_e + kEChunk - 1) / kEChunk;
+  P.n_echunks = (n_e + energies_per_chunk() - 1) / energies_per_chunk();
   P.atmosphere = st.atmosphere;
+  P.nsx_rows = nsx_rows;
 
   if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");
   if ((e = launch_ring_setup(c->L, c->N, P, c->stream))) return hip_fail(c, e, "ring set-up launch");
@@ -328,7 +355,7 @@ ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const s
   if (n_tasks == 0) {
     if ((e = hipMemsetAsync(d_flux, 0, sizeof(double) * g.n_phase * n_e, c->stream)))
       return hip_fail(c, e, "clear");
-  } else if ((e = launch_reduce(P, nullptr, c->stream))) {
+  } else if ((e = launch_reduce(P, (double*)W.red.p, c->stream))) {
     return hip_fail(c, e, "reduce launch");
   }
   if ((e = hipEventRecord(c->ev1, c->stream))) return hip_fail(c, e, "event");
@@ -347,6 +374,13 @@ ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const s
   if (n_tasks && (e = hipMemcpy(desc.data(), W.desc.p, sizeof(RingDesc) * n_tasks, hipMemcpyDeviceToHost)))
     return hip_fail(c, e, "descriptor read-back");
   int64_t n_act = 0, n_eval = 0, n_rings = 0;
+  bool rows_overflow = false;
+  for (const RingDesc& d : desc) rows_overflow |= (d.flags & 2) != 0;
+  if (rows_overflow && nsx_rows < c->N.nE) {
+    // a ring's redshift range needs more NSX rows than were staged: redo
+    // with the whole slice (results of the first pass are discarded)
+    return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, c->N.nE);
+  }
   for (const RingDesc& d : desc) {
     if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
     n_act += d.n_act;

@@ -35,9 +35,9 @@ struct HostBuf {
 };
 
 struct Workspace {
-  DevBuf spots, tasks, desc, ph, fine, pts, energies, partial, scale, out_flux, out_counts;
+  DevBuf spots, tasks, desc, ph, fine, pts, energies, partial, red, scale, out_flux, out_counts;
   void release_all() {
-    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &partial, &scale, &out_flux,
+    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &partial, &red, &scale, &out_flux,
                       &out_counts})
       b->release();
   }

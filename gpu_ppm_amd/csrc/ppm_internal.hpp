@@ -55,7 +55,7 @@ struct RingDesc {
   int n_pix, s_shift2;        // pixels in the ring, 2*s of phi_j = 2pi (j - s/2)/n_pix
   int n_act, n_runs;          // active patches and runs
   int kept;                   // 1 if the ring contributes
-  int flags;                  // bit0: run overflow
+  int flags;                  // bit0: run overflow, bit1: NSX rows beyond the staged slice
   int iT, ig;                 // NSX (logT, logg) bracket
   float aT, ag;
   int run_start[kMaxRuns];    // first active pixel j (1-based) of each run
@@ -103,16 +103,19 @@ struct PlainLaunch {
   double* d_flux;        // [n_phase][n_e] output (one star)
   int n_tasks, n_fine, n_pts, n_phase, n_e, n_echunks;
   int atmosphere;
+  int nsx_rows;          // NSX slice rows staged per workgroup
 };
 
 // Kernel launchers (ppm_kernels.hip).  Return a hipError_t as int.
 int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream);
 int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream);
-int launch_reduce(const PlainLaunch& P, const int* d_task_order, void* stream);
+int launch_reduce(const PlainLaunch& P, double* d_red, void* stream);
+int reduce_groups(int n_tasks);
 int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_e,
                 const double* d_flux, int n_batch, int n_phase, const double* d_scale,
                 double* d_counts, void* stream);
 int max_dynamic_lds(int device);
 size_t integral_lds_bytes(int n_fine, int atmosphere, int nsx_rows);
+int energies_per_chunk();
 
 }  // namespace ppm

@@ -15,6 +15,7 @@
 
 #include <cstdint>
 
+#include "ppm_eval.hpp"
 #include "ppm_internal.hpp"
 
 namespace ppm {
@@ -25,8 +26,6 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
 // ------------------------------------------------------------------ helpers
-
-__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // HEALPix ring i of N_side n: z, pixel count, 2*s (cpu/grid.hpp:106-131).
 __device__ inline void healpix_ring(int n, int i, double& z, int& n_pix, int& s2) {
@@ -66,24 +65,6 @@ __device__ inline int uniform_hunt(double x_min, double x_max, int n, double x, 
   const double t = (x - x_min) / (x_max - x_min) * (n - 1);
   const int i = (int)t;
   a = t - i;
-  return i;
-}
-
-// Bracket search with VecHunt's result (cpu/hunt.hpp:8-76): lower index of
-// the bracket, clamped to [0, n-2]; linear fraction extrapolates.
-template <typename T>
-__device__ inline int vec_hunt(const T* xx, int n, double x, double& a) {
-  const bool ascnd = xx[n - 1] >= xx[0];
-  int jl = 0, ju = n - 1;
-  while (ju - jl > 1) {
-    const int jm = (ju + jl) >> 1;
-    if ((x >= (double)xx[jm]) == ascnd)
-      jl = jm;
-    else
-      ju = jm;
-  }
-  const int i = clampi(jl, 0, n - 2);
-  a = (x - (double)xx[i]) / ((double)xx[i + 1] - (double)xx[i]);
   return i;
 }
 
@@ -426,18 +407,9 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
 
 // ---------------------------------------------------------- plain integral
 
-// Per-patch record produced by one lane and read by the whole wavefront.
-struct PatchRec {
-  float c;      // flux/I x GL weight (0: skip)
-  float tx;     // energy-axis offset of this patch, local to the chunk base
-  float zf;     // redshift factor (blackbody)
-  int im;       // first mu node of the stencil
-  float wm[4];  // mu weights
-};
-
-// Byte offsets of the integral kernel's dynamic LDS.
+// Byte offsets of the plain integral kernel's dynamic LDS.
 struct LdsLayout {
-  size_t ph, mu, fI, zf, cs, rec, S, total;
+  size_t ph, mu, fI, zf, cs, S, total;
 };
 __host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
   LdsLayout l;
@@ -453,24 +425,23 @@ __host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
   l.cs = o;
   o += (size_t)nf1 * sizeof(float);
   o = (o + 15) & ~(size_t)15;
-  l.rec = o;
-  o += (size_t)kBlock * sizeof(PatchRec);
   l.S = o;
   o += (size_t)nsx_rows * kSliceStride * sizeof(float);
   l.total = (o + 15) & ~(size_t)15;
   return l;
 }
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+constexpr int kEPL = 2;                     // energies per lane
+constexpr int kEPerWave = 64 * kEPL;        // energies per wavefront / workgroup
 
+// One workgroup per (ring, 128-energy chunk); its 4 wavefronts take the
+// phase bins round-robin.  Within a wavefront each lane sets up one patch in
+// FP64 and keeps it in registers; the evaluation loop broadcasts a patch
+// with readlane and every lane evaluates it at its two energies.
 template <int ATM>
 __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx N, PlainLaunch P) {
   const int task = blockIdx.x, chunk = blockIdx.y;
-  const RingDesc* Dp = P.d_desc + task;
+  RingDesc* Dp = P.d_desc + task;
   if (!Dp->kept) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nf1 = P.n_fine + 1;
@@ -478,16 +449,15 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
   const SpotParams& S = P.d_spots[T.spot];
 
   extern __shared__ __align__(16) unsigned char smem[];
-  const LdsLayout lay = integral_layout(nf1, ATM == PPM_ATM_NSX ? N.nE : 0);
+  const int rows_alloc = ATM == PPM_ATM_NSX ? P.nsx_rows : 0;
+  const LdsLayout lay = integral_layout(nf1, rows_alloc);
   double* s_ph = reinterpret_cast<double*>(smem + lay.ph);
   double* s_mu = reinterpret_cast<double*>(smem + lay.mu);
   float* s_fI = reinterpret_cast<float*>(smem + lay.fI);
   float* s_zf = reinterpret_cast<float*>(smem + lay.zf);
   float* s_cs = reinterpret_cast<float*>(smem + lay.cs);
-  PatchRec* s_rec = reinterpret_cast<PatchRec*>(smem + lay.rec);
   float* s_S = reinterpret_cast<float*>(smem + lay.S);
 
-  // ---- stage the ring's fine tables and the NSX slice
   {
     const double* gph = P.d_ph + (size_t)task * nf1;
     const float* gf = P.d_fine + (size_t)task * 3 * nf1;
@@ -498,69 +468,63 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
       s_cs[k] = gf[2 * nf1 + k];
     }
   }
+  // energies of this chunk and the local energy axis
+  const int e_first = chunk * kEPerWave;
+  const int e_last = min(e_first + kEPerWave, P.n_e) - 1;
+  const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
+  const double inv_h = 1.0 / h;
+  const double l_ref = log10(P.d_energies[e_first]);
+  int r_lo = 0, n_rows = 0;
+  EAxis ax{0.f, 0.f, 0, 0};
+  if (ATM == PPM_ATM_NSX) {
+    const double lkT = log10(S.kT_nsx);
+    const double t_min = (l_ref + log10(Dp->zf_min) - lkT - N.logE_min) * inv_h;
+    const double t_max = (log10(P.d_energies[e_last]) + log10(Dp->zf_max) - lkT - N.logE_min) * inv_h;
+    r_lo = clampi((int)floor(t_min - 0.01) - 1, 0, N.nE - 4);
+    const int r_hi = clampi((int)floor(t_max + 0.01) - 1, 0, N.nE - 4) + 3;
+    n_rows = r_hi - r_lo + 1;
+    if (n_rows > rows_alloc) {  // redshift range beyond the staged rows: flagged, host re-runs
+      if (tid == 0) atomicOr(&Dp->flags, 2);
+      n_rows = rows_alloc;
+    }
+    ax.t_lo = (float)(-r_lo);
+    ax.t_hi = (float)(N.nE - 1 - r_lo);
+    ax.st_lo = 0;
+    ax.st_hi = min(N.nE - 4 - r_lo, n_rows - 4);
+    for (int k = tid; k < N.nm; k += kBlock) s_mu[k] = N.mu[k];
+    stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBlock);
+  }
+  __syncthreads();
+  const double tx_base = (l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo;
+
+  float le[kEPL], Ef[kEPL];
+  bool ok[kEPL];
+#pragma unroll
+  for (int u = 0; u < kEPL; ++u) {
+    const int e = e_first + u * 64 + lane;
+    ok[u] = e <= e_last;
+    const double E = P.d_energies[ok[u] ? e : e_last];
+    Ef[u] = (float)E;
+    le[u] = (float)((log10(E) - l_ref) * inv_h);
+  }
+  const float inv_kT = (float)S.inv_kT;
   const int n_act = Dp->n_act, n_runs = Dp->n_runs, n_pix = Dp->n_pix, s2 = Dp->s_shift2;
   const bool point = T.ring == 0;
   const double ps_shift = Dp->ps_shift;
-  const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
-  if (ATM == PPM_ATM_NSX) {
-    for (int k = tid; k < N.nm; k += kBlock) s_mu[k] = N.mu[k];
-    const int iT = Dp->iT, ig = Dp->ig;
-    const float aT = Dp->aT, ag = Dp->ag;
-    const float w00 = (1.f - aT) * (1.f - ag), w01 = (1.f - aT) * ag, w10 = aT * (1.f - ag),
-                w11 = aT * ag;
-    const size_t sT = (size_t)N.ng * N.nE * N.nm, sg = (size_t)N.nE * N.nm;
-    const float* c00 = N.data + iT * sT + ig * sg;
-    const float* c01 = c00 + sg;
-    const float* c10 = c00 + sT;
-    const float* c11 = c10 + sg;
-    const int total = N.nE * kSliceStride;
-    for (int k = tid; k < total; k += kBlock) {
-      const int r = k / kSliceStride, m = k - r * kSliceStride;
-      float v = 0.f;
-      if (m < N.nm) {
-        const size_t o = (size_t)r * N.nm + m;
-        v = w00 * c00[o] + w01 * c01[o] + w10 * c10[o] + w11 * c11[o];
-      }
-      s_S[k] = v;
-    }
-  }
-  __syncthreads();
-
-  // ---- per-lane energy
-  const int e = chunk * kEChunk + lane;
-  const bool e_ok = e < P.n_e;
-  const double E = P.d_energies[e_ok ? e : P.n_e - 1];
-  // energy axis position t = (log10(E zf / kT) - logE_min) / h is split into
-  // a lane part (le) and a patch part (tx) around an integer base r0 so both
-  // stay small in FP32
-  const double E_mid = P.d_energies[min(chunk * kEChunk + 32, P.n_e - 1)];
-  const double zf_mid = 0.5 * (Dp->zf_min + Dp->zf_max);
-  const double t_mid = (log10(E_mid * zf_mid / S.kT_nsx) - N.logE_min) / h;
-  const int r0 = ATM == PPM_ATM_NSX ? clampi((int)floor(t_mid), 0, N.nE - 1) : 0;
-  const float le = (float)((log10(E) - log10(E_mid)) / h);
-  const double tx_base = (log10(E_mid) - log10(S.kT_nsx) - N.logE_min) / h - r0;
-  const float t_lo = (float)(-r0), t_hi = (float)(N.nE - 1 - r0);
-  const int st_lo = -r0, st_hi = N.nE - 4 - r0;
-  const float Ef = (float)E;
-  const float inv_kT = (float)S.inv_kT;
-  const float* s_Sr0 = s_S + r0 * kSliceStride;
-
-  PatchRec* rec = s_rec + wave * 64;
+  const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
   const int ng = P.n_pts / P.n_phase;
 
   for (int b = wave; b < P.n_phase; b += kWaves) {
-    float acc = 0.f, comp = 0.f;
+    float acc[kEPL], comp[kEPL];
+#pragma unroll
+    for (int u = 0; u < kEPL; ++u) acc[u] = comp[u] = 0.f;
     for (int g = 0; g < ng; ++g) {
       const PhasePoint pt = P.d_pts[b * ng + g];
       for (int base = 0; base < n_act; base += 64) {
-        // ---- patch set-up, one patch per lane, FP64
+        // ---- patch set-up, one patch per lane (FP64)
+        PatchVals pv;
+        pv.im = -1;
         const int k = base + lane;
-        PatchRec r;
-        r.c = 0.f;
-        r.tx = 0.f;
-        r.zf = 0.f;
-        r.im = 0;
-        r.wm[0] = r.wm[1] = r.wm[2] = r.wm[3] = 0.f;
         if (k < n_act) {
           double shift;
           if (point) {
@@ -575,107 +539,78 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
               }
               kk -= len;
             }
-            shift = (j - 0.5 * s2 * 0.5) / n_pix;
+            shift = (j - 0.25 * s2) / n_pix;
           }
           // time-delayed phase (cpu/devarshi24st_cpu.cpp:218-224)
           const double v = pt.p + shift + 1.;
           double pp = v - floor(v);
           if (pp < s_ph[0]) pp += 1.;
-          double a;
-          const int i = vec_hunt(s_ph, nf1, pp, a);
-          const double fI = (1 - a) * (double)s_fI[i] + a * (double)s_fI[i + 1];
-          if (fI != 0) {
-            const double zz = (1 - a) * (double)s_zf[i] + a * (double)s_zf[i + 1];
-            r.c = (float)(fI * pt.w);
-            r.zf = (float)zz;
-            if (ATM == PPM_ATM_NSX) {
-              double mu = (1 - a) * (double)s_cs[i] + a * (double)s_cs[i + 1];
-              if (mu < N.mu_min) mu = N.mu_min;
-              double am;
-              int im = vec_hunt(s_mu, N.nm, mu, am);
-              if (im >= N.nm - 5) {  // linear at the edge (cpu/nsx.hpp:272-276)
-                r.wm[0] = (float)(1 - am);
-                r.wm[1] = (float)am;
-              } else {  // cubic (cpu/nsx.hpp:277-281)
-                im = clampi(im - 1, 0, N.nm - 4);
-                const double x0 = s_mu[im], x1 = s_mu[im + 1], x2 = s_mu[im + 2], x3 = s_mu[im + 3];
-                r.wm[0] = (float)((mu - x1) * (mu - x2) * (mu - x3) / ((x0 - x1) * (x0 - x2) * (x0 - x3)));
-                r.wm[1] = (float)((mu - x0) * (mu - x2) * (mu - x3) / ((x1 - x0) * (x1 - x2) * (x1 - x3)));
-                r.wm[2] = (float)((mu - x0) * (mu - x1) * (mu - x3) / ((x2 - x0) * (x2 - x1) * (x2 - x3)));
-                r.wm[3] = (float)((mu - x0) * (mu - x1) * (mu - x2) / ((x3 - x0) * (x3 - x1) * (x3 - x2)));
-              }
-              r.im = im;
-              r.tx = (float)(tx_base + log10(zz) / h);
-            }
-          }
+          patch_setup<ATM>(pp, pt.w, R, s_mu, N.nm, N.mu_min, tx_base, inv_h, pv);
         }
-        rec[lane] = r;
-        wave_sync();
-        // ---- evaluation, one energy per lane
+        // ---- evaluation: every lane evaluates each patch at its energies
         const int nk = min(64, n_act - base);
-        float bsum = 0.f;
-        for (int q = 0; q < nk; ++q) {
-          const PatchRec& R = rec[q];
-          if (R.c == 0.f) continue;
-          float I;
-          if (ATM == PPM_ATM_NSX) {
-            float t = le + R.tx;
-            t = fminf(fmaxf(t, t_lo), t_hi);
-            const int st = clampi((int)floorf(t) - 1, st_lo, st_hi);
-            const float vv = t - (float)st;
-            const float a0 = vv, a1 = vv - 1.f, a2 = vv - 2.f, a3 = vv - 3.f;
-            const float w0 = -(a1 * a2 * a3) * (1.f / 6.f);
-            const float w1 = (a0 * a2 * a3) * 0.5f;
-            const float w2 = -(a0 * a1 * a3) * 0.5f;
-            const float w3 = (a0 * a1 * a2) * (1.f / 6.f);
-            const float* row = s_Sr0 + st * kSliceStride + R.im;
-            float rr[4];
+        float bsum[kEPL];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-              const float* rw = row + kk * kSliceStride;
-              rr[kk] = R.wm[0] * rw[0] + R.wm[1] * rw[1] + R.wm[2] * rw[2] + R.wm[3] * rw[3];
-            }
-            I = w0 * rr[0] + w1 * rr[1] + w2 * rr[2] + w3 * rr[3];
+        for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
+        for (int q = 0; q < nk; ++q) {
+          const int im = readlane_i(pv.im, q);
+          if (im < 0) continue;
+          if (ATM == PPM_ATM_NSX) {
+            const float tx = readlane_f(pv.tx, q);
+            const float w[4] = {readlane_f(pv.w[0], q), readlane_f(pv.w[1], q), readlane_f(pv.w[2], q),
+                                readlane_f(pv.w[3], q)};
+            const float* rows = s_S + im;
+#pragma unroll
+            for (int u = 0; u < kEPL; ++u) bsum[u] += nsx_eval(le[u] + tx, ax, rows, w);
           } else {
-            const float Ee = Ef * R.zf;
-            I = Ee * Ee * Ee / expm1f(Ee * inv_kT);
+            const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
+#pragma unroll
+            for (int u = 0; u < kEPL; ++u) {
+              const float Ee = Ef[u] * z;
+              bsum[u] = fmaf(c, Ee * Ee * Ee / expm1f(Ee * inv_kT), bsum[u]);
+            }
           }
-          bsum += R.c * I;
         }
-        wave_sync();
         // Kahan-compensated per-bin accumulation
-        const float y = bsum - comp;
-        const float tt = acc + y;
-        comp = (tt - acc) - y;
-        acc = tt;
+#pragma unroll
+        for (int u = 0; u < kEPL; ++u) {
+          const float y = bsum[u] - comp[u];
+          const float tt = acc[u] + y;
+          comp[u] = (tt - acc[u]) - y;
+          acc[u] = tt;
+        }
       }
     }
-    if (e_ok) P.d_partial[((size_t)task * P.n_phase + b) * P.n_e + e] = acc - comp;
+#pragma unroll
+    for (int u = 0; u < kEPL; ++u)
+      if (ok[u]) P.d_partial[((size_t)task * P.n_phase + b) * P.n_e + e_first + u * 64 + lane] = acc[u] - comp[u];
   }
 }
 
 // ------------------------------------------------------------------ reduce
 
-__global__ void k_reduce(PlainLaunch P) {
+// Stage 1: group g sums its contiguous range of ring tasks, each term scaled
+// by its spot's FP64 constant.  Stage 2 sums the groups in order and divides
+// by E (cpu/devarshi24st_cpu.cpp:238).  Fixed order: deterministic.
+__global__ void k_reduce_groups(PlainLaunch P, double* red, int n_groups) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int n_out = P.n_phase * P.n_e;
   if (idx >= n_out) return;
-  const int e = idx % P.n_e;
-  double total = 0.0, part = 0.0;
-  int spot = -1;
-  double scale = 0.0;
-  for (int t = 0; t < P.n_tasks; ++t) {
-    const int s = P.d_tasks[t].spot;
-    if (s != spot) {
-      total += part * scale;
-      part = 0.0;
-      spot = s;
-      scale = P.d_spots[s].scale;
-    }
-    if (P.d_desc[t].kept) part += (double)P.d_partial[(size_t)t * n_out + idx];
-  }
-  total += part * scale;
-  P.d_flux[idx] = total / P.d_energies[e];
+  const int g = blockIdx.y;
+  const int t0 = (int)((long long)P.n_tasks * g / n_groups), t1 = (int)((long long)P.n_tasks * (g + 1) / n_groups);
+  double total = 0.0;
+  for (int t = t0; t < t1; ++t)
+    if (P.d_desc[t].kept) total += P.d_spots[P.d_tasks[t].spot].scale * (double)P.d_partial[(size_t)t * n_out + idx];
+  red[(size_t)g * n_out + idx] = total;
+}
+
+__global__ void k_reduce_final(PlainLaunch P, const double* red, int n_groups) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_out = P.n_phase * P.n_e;
+  if (idx >= n_out) return;
+  double total = 0.0;
+  for (int g = 0; g < n_groups; ++g) total += red[(size_t)g * n_out + idx];
+  P.d_flux[idx] = total / P.d_energies[idx % P.n_e];
 }
 
 // ------------------------------------------------------------------- fold
@@ -744,6 +679,8 @@ size_t integral_lds_bytes(int n_fine, int atmosphere, int nsx_rows) {
   return integral_layout(n_fine + 1, atmosphere == PPM_ATM_NSX ? nsx_rows : 0).total;
 }
 
+int energies_per_chunk() { return kEPerWave; }
+
 int max_dynamic_lds(int device) {
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
@@ -759,7 +696,7 @@ int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P
 
 int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream) {
   if (P.n_tasks == 0) return hipSuccess;
-  const size_t lds = integral_lds_bytes(P.n_fine, P.atmosphere, N.nE);
+  const size_t lds = integral_lds_bytes(P.n_fine, P.atmosphere, P.nsx_rows);
   dim3 grid(P.n_tasks, P.n_echunks);
   if (P.atmosphere == PPM_ATM_NSX) {
     hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<PPM_ATM_NSX>,
@@ -775,9 +712,14 @@ int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunc
   return hipGetLastError();
 }
 
-int launch_reduce(const PlainLaunch& P, const int*, void* stream) {
+int reduce_groups(int n_tasks) { return n_tasks < 32 ? 1 : (n_tasks / 16 < 64 ? n_tasks / 16 : 64); }
+
+int launch_reduce(const PlainLaunch& P, double* d_red, void* stream) {
   const int n = P.n_phase * P.n_e;
-  hipLaunchKernelGGL(k_reduce, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, P);
+  const int G = reduce_groups(P.n_tasks);
+  hipLaunchKernelGGL(k_reduce_groups, dim3((n + 63) / 64, G), dim3(64), 0, (hipStream_t)stream, P, d_red, G);
+  hipLaunchKernelGGL(k_reduce_final, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, P,
+                     (const double*)d_red, G);
   return hipGetLastError();
 }
 

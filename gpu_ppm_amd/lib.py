@@ -168,7 +168,7 @@ class Context:
         return flux, cnt, stats
 
     def eval_device(self, stars, grid, d_flux, d_counts=None):
-        """Outputs to device pointers (ints, e.g. torch tensor data_ptr())."""
+        """Outputs to device pointers (ints, e.g. gpu_ppm_amd.hip.DeviceArray.ptr)."""
         arr = _star_array(stars)
         stats = abi.Stats()
         self._check(load().ppm_eval_device(self._h, arr, len(arr), byref(grid), c_void_p(d_flux),
