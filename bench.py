@@ -1,22 +1,24 @@
 """Benchmark of the pulse-profile path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched|plain]
 
 Prints one JSON line.  A step is one pass of the hot path over one batch of
-synthetic input with inputs and outputs resident in HBM; the timed region is
-K steps between two device synchronisations.
+synthetic input with tables, inputs and outputs resident in HBM; the timed
+region is K steps between two device synchronisations.
 
 Workloads (the configurations the headline metric is quoted on):
-  plain    one oblate-star profile (single 0.25 rad cap at 20 deg, NSX-format
-           atmosphere, kT 0.35 keV) at N_side 1024, 64 phase bins x 2 GL
-           nodes, 360 energies, folded through a 300 x 360 response; one
-           profile per launch sequence, as the reference's calc_one_star.
-  batched  256 stars drawn from the batched-evaluation priors at the same
-           resolution in one persistent launch (when built).
+  batched (default)  256 stars drawn (seed 0) from the batched-evaluation
+           priors: M 1.0-2.2 Msun, R_eq 9-15 km, spin 200-700 Hz,
+           inclination 5-90 deg, one cap of colatitude 0-180 deg and radius
+           0.02-1.2 rad, kT 0.1-0.5 keV; NSX-format atmosphere; N_side 1024,
+           64 phase bins x 2 GL nodes, 360 energies, 300 x 360 response fold.
+           One step = the 256 profiles in one persistent launch (hipGraph).
+           The plain path is timed on the same stars, one profile per launch.
+  plain    one star (0.25 rad cap at 20 deg, M 1.4, R_eq 12 km, 600 Hz,
+           i 60 deg, kT 0.35 keV) at the same resolution, one profile per step.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -26,8 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "profile evals/s, batched vs plain GPU, at ≤1e-4 max rel err; p50/p99 latency"
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak
-HBM_PEAK_GBS = 8000.0
 FLOP_PER_EVAL = 32         # 4 x 4 (logEkT x mu) Lagrange stencil = 16 FMA
+N_SIDE, N_PHASE, N_E = 1024, 64, 360
 
 
 def log(*a):
@@ -41,34 +43,100 @@ def data_root():
     return root
 
 
-def cpu_baseline(root, star, grid, target_s=15.0):
+def grid(method):
+    from gpu_ppm_amd import abi
+    return abi.make_grid(N_SIDE, N_PHASE, sampling=abi.SAMPLE_GL, n_gl=2, response=0, method=method)
+
+
+def cpu_baseline(root, stars, lensing, target_s=20.0):
     """The FP64 oracle (a C port of the reference CPU path) on a bounded
-    sample of the same workload: all rings and phase bins, a subset of the
-    energies; the per-profile time is scaled by the energy ratio."""
+    sample of the same workload: the first star(s), all rings and phase bins,
+    a subset of the energies; the per-profile time is scaled by the energy
+    ratio."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from gpu_ppm_amd import abi
     from gpu_ppm_amd import scenarios as S
     from oracle_binding import Oracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    orc = Oracle(root, nsx_path=os.path.join(root, "ext", "model_data", "nsx_H_v200804.out"), responses=("std",))
-    energies = S.geomspace(0.2, 12.0, 360)
+    orc = Oracle(root, lensing=lensing, nsx_path=os.path.join(root, "ext", "model_data", "nsx_H_v200804.out"),
+                 responses=("std",))
+    energies = S.geomspace(0.2, 12.0, N_E)
 
-    def run(n_e):
-        idx = [int(round(k * 359 / max(n_e - 1, 1))) for k in range(n_e)]
-        g = abi.make_grid(grid.n_side, grid.n_phase, n_fine=grid.n_fine, sampling=grid.sampling,
-                          n_gl=grid.n_gl, energies=[energies[i] for i in idx])
+    def run(n_e, sts):
+        idx = [int(round(k * (N_E - 1) / max(n_e - 1, 1))) for k in range(n_e)]
+        g = abi.make_grid(N_SIDE, N_PHASE, sampling=abi.SAMPLE_GL, n_gl=2, energies=[energies[i] for i in idx])
         t0 = time.perf_counter()
-        orc.profile(star, g, threads=threads)
+        for st in sts:
+            orc.profile(st, g, threads=threads)
         return time.perf_counter() - t0
 
-    t2 = run(2)
-    n_e = max(2, min(360, int(2 * target_s / max(t2, 1e-3))))
-    t = run(n_e)
+    sample = stars[:2]
+    t2 = run(2, sample)
+    n_e = max(2, min(N_E, int(2 * target_s / max(t2, 1e-3))))
+    t = run(n_e, sample)
     orc.close()
-    per_profile = t * 360.0 / n_e
+    per_profile = t * N_E / n_e / len(sample)
     return {"value": 1.0 / per_profile, "unit": "profiles/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle, same star and grid, all rings and bins, %d of 360 energies "
-                      "in %.1f s on %d threads; scaled by 360/%d" % (n_e, t, threads, n_e)}
+            "sample": "FP64 oracle on %d of the workload's stars, all rings and bins, %d of %d energies, "
+                      "%.1f s on %d threads; scaled by %d/%d" % (len(sample), n_e, N_E, t, threads, N_E, n_e)}
+
+
+def timed(ctx, stars, g, d_flux, d_cnt, steps, warmup):
+    from gpu_ppm_amd import hip
+    for _ in range(warmup):
+        ctx.eval_device(stars, g, d_flux.ptr, d_cnt.ptr)
+    hip.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.eval_device(stars, g, d_flux.ptr, d_cnt.ptr))
+    hip.synchronize()
+    return time.perf_counter() - t0, stats
+
+
+def bench_batched(root, steps, warmup, n_stars, plain_stars):
+    from gpu_ppm_amd import abi, hip
+    from gpu_ppm_amd import scenarios as S
+    from gpu_ppm_amd.lib import Context
+    stars = S.sample_priors(n_stars, seed=0)
+    ctx = Context(root, lensing="wide", responses=("std",))
+    n_ch, e = ctx.response_info(0)
+    d_flux = hip.DeviceArray((n_stars, N_PHASE, N_E))
+    d_cnt = hip.DeviceArray((n_stars, n_ch, N_PHASE))
+    gb = grid(abi.METHOD_BATCHED)
+    log("bench: batched %d stars, warmup %d, steps %d" % (n_stars, warmup, steps))
+    dt, st = timed(ctx, stars, gb, d_flux, d_cnt, steps, warmup)
+    log("bench: batched %.3f s per step" % (dt / steps))
+    ctx.close()
+    # per-kernel time of the batched integral: one direct (non-graph) launch
+    os.environ["PPM_NO_GRAPH"] = "1"
+    ctx = Context(root, lensing="wide", responses=("std",))
+    _, sk = timed(ctx, stars, gb, d_flux, d_cnt, 1, 0)
+    # plain path on the same stars, one profile per launch sequence
+    gp = grid(abi.METHOD_PLAIN)
+    sub = stars[:plain_stars]
+    d_f1 = hip.DeviceArray((1, N_PHASE, N_E))
+    d_c1 = hip.DeviceArray((1, n_ch, N_PHASE))
+    ctx.eval_device([sub[0]], gp, d_f1.ptr, d_c1.ptr)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    ref_evals, plain_int_ms = 0, 0.0
+    for s in sub:
+        ps = ctx.eval_device([s], gp, d_f1.ptr, d_c1.ptr)
+        ref_evals += ps.n_eval_ref
+        plain_int_ms += ps.integral_ms
+    hip.synchronize()
+    t_plain = time.perf_counter() - t0
+    log("bench: plain %d stars in %.2f s" % (len(sub), t_plain))
+    ctx.close()
+    del os.environ["PPM_NO_GRAPH"]
+    k = sk[0]
+    return dict(stars=stars, lensing="wide", value=n_stars * steps / dt, dt=dt,
+                kernel="k_integral_batched", kernel_ms=k.integral_ms, n_eval=k.n_eval_done,
+                n_eval_ref=k.n_eval_ref, n_active=k.n_active,
+                plain_profiles_per_s=len(sub) / t_plain, plain_stars=len(sub),
+                plain_evals_per_s=ref_evals / (plain_int_ms * 1e-3),
+                graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms)
 
 
 def bench_plain(root, steps, warmup):
@@ -76,50 +144,48 @@ def bench_plain(root, steps, warmup):
     from gpu_ppm_amd import scenarios as S
     from gpu_ppm_amd.lib import Context
     star = S.cfg1_star()
-    grid = abi.make_grid(1024, 64, sampling=abi.SAMPLE_GL, n_gl=2, response=0, method=abi.METHOD_PLAIN)
     ctx = Context(root, lensing="std", responses=("std",))
     n_ch, e = ctx.response_info(0)
-    d_flux = hip.DeviceArray((1, 64, e.size))
-    d_cnt = hip.DeviceArray((1, n_ch, 64))
-    hip.synchronize()
-    for _ in range(warmup):
-        ctx.eval_device(star, grid, d_flux.ptr, d_cnt.ptr)
-    hip.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        stats.append(ctx.eval_device(star, grid, d_flux.ptr, d_cnt.ptr))
-    hip.synchronize()
-    dt = time.perf_counter() - t0
-    integral_ms = sum(s.integral_ms for s in stats) / steps
-    n_eval = stats[0].n_eval_done
+    d_flux = hip.DeviceArray((1, N_PHASE, N_E))
+    d_cnt = hip.DeviceArray((1, n_ch, N_PHASE))
+    dt, st = timed(ctx, [star], grid(abi.METHOD_PLAIN), d_flux, d_cnt, steps, warmup)
     ctx.close()
-    return dict(star=star, grid=grid, dt=dt, profiles=steps, integral_ms=integral_ms, n_eval=n_eval,
-                n_eval_ref=stats[0].n_eval_ref, n_active=stats[0].n_active, kernel="k_integral_plain",
-                kernel_ms=sum(s.kernel_ms for s in stats) / steps, fold_ms=sum(s.fold_ms for s in stats) / steps)
+    return dict(stars=[star], lensing="std", value=steps / dt, dt=dt, kernel="k_integral_plain",
+                kernel_ms=sum(s.integral_ms for s in st) / steps, n_eval=st[0].n_eval_done,
+                n_eval_ref=st[0].n_eval_ref, n_active=st[0].n_active)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--stars", type=int, default=256, help="stars per batched step")
+    ap.add_argument("--plain-stars", type=int, default=64, help="stars timed on the plain path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (profiles shard by "
-              "star or ring range with one small reduction, see the design notes); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (profiles shard by star or by "
+              "ring range with one small reduction); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     root = data_root()
     log("bench: data ready; workload", args.workload)
-    r = bench_plain(root, args.steps, args.warmup)
-    log("bench: timed %d steps in %.3f s" % (args.steps, r["dt"]))
-    achieved = r["n_eval"] * FLOP_PER_EVAL / (r["integral_ms"] * 1e-3) / 1e12
+    if args.workload == "batched":
+        r = bench_batched(root, args.steps, args.warmup, args.stars, args.plain_stars)
+        cfg = {"workload": "batched: %d prior-drawn stars per launch, N_side 1024, 64 bins x 2 GL, 360 E, NSX, "
+                           "fold 300x360" % args.stars, "n_stars": args.stars}
+        unit_note = "%d distinct evaluations (phase-shift dedupe) x %d FLOP" % (r["n_eval"], FLOP_PER_EVAL)
+    else:
+        r = bench_plain(root, args.steps, args.warmup)
+        cfg = {"workload": "plain single profile: N_side 1024, 64 bins x 2 GL, 360 E, NSX, fold 300x360"}
+        unit_note = "%d evaluations x %d FLOP" % (r["n_eval"], FLOP_PER_EVAL)
+    cfg.update({"n_side": N_SIDE, "n_phase": N_PHASE, "n_gl": 2, "n_energy": N_E, "n_active": r["n_active"]})
+    achieved = r["n_eval"] * FLOP_PER_EVAL / (r["kernel_ms"] * 1e-3) / 1e12
+    ref_eq = r["n_eval_ref"] * FLOP_PER_EVAL / (r["kernel_ms"] * 1e-3) / 1e12
     out = {
         "metric": METRIC,
-        "value": r["profiles"] / r["dt"],
+        "value": r["value"],
         "unit": "profiles/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -130,19 +196,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain single-profile: N_side 1024, 64 bins x 2 GL, 360 E, NSX, fold 300x360",
-                   "n_side": 1024, "n_phase": 64, "n_energy": 360, "n_active": r["n_active"],
-                   "method": "plain"},
+        "config": cfg,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": r["kernel"],
-                     "kernel_ms": r["integral_ms"], "roof": "FP32 vector (no packed math)",
-                     "units": "%d evaluations x %d FLOP" % (r["n_eval"], FLOP_PER_EVAL)},
-        "plain": {"profiles_per_s": r["profiles"] / r["dt"], "kernel_ms": r["kernel_ms"],
-                  "integral_ms": r["integral_ms"], "fold_ms": r["fold_ms"],
-                  "n_eval_ref": r["n_eval_ref"], "evals_per_s": r["n_eval"] / (r["integral_ms"] * 1e-3)},
+                     "kernel_ms": r["kernel_ms"], "roof": "FP32 vector ALU (no packed FP32)",
+                     "units": unit_note, "ref_equivalent_tflops": ref_eq},
     }
+    if args.workload == "batched":
+        out["plain"] = {"profiles_per_s": r["plain_profiles_per_s"], "stars_timed": r["plain_stars"],
+                        "evals_per_s_ref": r["plain_evals_per_s"]}
+        out["batched_vs_plain"] = r["value"] / r["plain_profiles_per_s"]
+        out["batched"] = {"graph_ms_per_step": r["graph_ms"], "integral_ms": r["kernel_ms"],
+                          "setup_reduce_fold_ms": r["setup_reduce_fold_ms"], "n_eval_ref": r["n_eval_ref"],
+                          "dedupe_factor": r["n_eval_ref"] / max(r["n_eval"], 1)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(root, r["star"], r["grid"])
+        out["cpu_baseline"] = cpu_baseline(root, r["stars"], r["lensing"])
     print(json.dumps(out), flush=True)
 
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -238,33 +239,24 @@ ppm_status resolve_energies(ppm_ctx* c, const ppm_grid& g, std::vector<double>& 
   return PPM_OK;
 }
 
-// Upload host arrays through the pinned staging buffer.
-struct Upload {
-  std::vector<std::pair<const void*, size_t>> src;
-  std::vector<void*> dst;
-  void add(void* d, const void* s, size_t n) {
-    src.emplace_back(s, n);
-    dst.push_back(d);
-  }
-  int run(ppm_ctx* c) {
-    size_t total = 0;
-    for (auto& s : src) total += (s.second + 255) & ~(size_t)255;
-    int e = c->staging.ensure(total);
+int Upload::run(ppm_ctx* c) {
+  size_t total = 0;
+  for (auto& s : src) total += (s.second + 255) & ~(size_t)255;
+  int e = c->staging.ensure(total);
+  if (e) return e;
+  // the previous upload from this buffer must have been consumed
+  e = hipEventSynchronize(c->ev_staging);
+  if (e) return e;
+  char* h = (char*)c->staging.p;
+  size_t off = 0;
+  for (size_t k = 0; k < src.size(); ++k) {
+    std::memcpy(h + off, src[k].first, src[k].second);
+    e = hipMemcpyAsync(dst[k], h + off, src[k].second, hipMemcpyHostToDevice, c->stream);
     if (e) return e;
-    // the previous upload from this buffer must have been consumed
-    e = hipEventSynchronize(c->ev_staging);
-    if (e) return e;
-    char* h = (char*)c->staging.p;
-    size_t off = 0;
-    for (size_t k = 0; k < src.size(); ++k) {
-      std::memcpy(h + off, src[k].first, src[k].second);
-      e = hipMemcpyAsync(dst[k], h + off, src[k].second, hipMemcpyHostToDevice, c->stream);
-      if (e) return e;
-      off += (src[k].second + 255) & ~(size_t)255;
-    }
-    return hipEventRecord(c->ev_staging, c->stream);
+    off += (src[k].second + 255) & ~(size_t)255;
   }
-};
+  return hipEventRecord(c->ev_staging, c->stream);
+}
 
 // One star, plain path: the reference iteration space with one launch
 // sequence per profile.  Outputs are device pointers.
@@ -520,6 +512,10 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
     return PPM_ERR_DEVICE;
   }
   c->max_lds = max_dynamic_lds(o.device);
+  {
+    const char* ng = std::getenv("PPM_NO_GRAPH");
+    c->no_graph = ng && *ng && *ng != '0';
+  }
   c->lens = std::move(lens);
   {  // lensing tables, FP64 on the device
     const LensingHost& h = c->lens;
@@ -602,6 +598,7 @@ ppm_status ppm_close(ppm_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->ws.release_all();
   c->batch.release_all();
+  release_batch_state(c);
   c->d_lens.release();
   c->d_nsx.release();
   c->d_mu.release();

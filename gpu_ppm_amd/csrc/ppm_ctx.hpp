@@ -45,10 +45,9 @@ struct Workspace {
 
 // Buffers of the batched path (ppm_batch.cpp).
 struct BatchWorkspace {
-  DevBuf spots, tasks, desc, ph, fine, pts, energies, items, flux_acc, scale, counter, rowmap;
+  DevBuf spots, tasks, desc, ph, fine, pts, energies, items, scale, counter, evals;
   void release_all() {
-    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &items, &flux_acc, &scale, &counter,
-                      &rowmap})
+    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &items, &scale, &counter, &evals})
       b->release();
   }
 };
@@ -63,6 +62,7 @@ struct Response {
 };
 
 struct ServeState;  // ppm_serve.cpp
+struct BatchState;  // ppm_batch.cpp
 
 inline uint16_t bf16_round(float f) {
   uint32_t u;
@@ -102,6 +102,8 @@ struct ppm_ctx {
   ppm::HostBuf staging;
 
   ppm::ServeState* serve = nullptr;
+  ppm::BatchState* batch_state = nullptr;
+  bool no_graph = false;  // PPM_NO_GRAPH=1: launch the batched sequence directly
 };
 
 namespace ppm {
@@ -118,5 +120,18 @@ ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
 
 void serve_init(ppm_ctx* c);
 void serve_shutdown(ppm_ctx* c);
+void release_batch_state(ppm_ctx* c);
+
+// Host arrays copied to the device through the context's pinned staging
+// buffer on its stream (ppm_api.cpp).
+struct Upload {
+  std::vector<std::pair<const void*, size_t>> src;
+  std::vector<void*> dst;
+  void add(void* d, const void* s, size_t n) {
+    src.emplace_back(s, n);
+    dst.push_back(d);
+  }
+  int run(ppm_ctx* c);
+};
 
 }  // namespace ppm

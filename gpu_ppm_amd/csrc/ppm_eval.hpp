@@ -129,6 +129,67 @@ __device__ inline float readlane_f(float v, int lane) {
 }
 __device__ inline int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows of the contracted slice one wavefront keeps per patch.
+constexpr int kRowsCap = 128;
+
+// Per-wavefront LDS scratch of the contracted evaluation.
+struct WaveRows {
+  float* U;    // [kRowsCap] slice rows contracted over the patch's mu stencil
+  float4* C;   // [kRowsCap] cubic coefficients (Newton form) starting at each row
+};
+
+// All energies of a wavefront at one patch (uniform: tx, im, w).  The mu
+// stencil is contracted once per row the wavefront touches, each row start
+// gets the Newton coefficients of the cubic through 4 consecutive rows, and
+// each energy is then one coefficient read and three FMAs:
+//   I(v) = r0 + v (a + (v-1) (d2/2 + (v-2) d3/6)),  v = t - st in [0, 3],
+// the same cubic Lagrange polynomial as cpu/nsx.hpp:159-167 on those rows.
+// Returns false (nothing written) when the rows exceed the scratch.
+template <int EPL>
+__device__ inline bool nsx_eval_rows(const float (&le)[EPL], float tx, int im, const float (&w)[4], const EAxis& ax,
+                                     const float* s_S, const WaveRows& wr, int lane, float (&out)[EPL]) {
+  int st[EPL];
+  float v[EPL];
+#pragma unroll
+  for (int u = 0; u < EPL; ++u) {
+    const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
+    st[u] = med3i((int)floorf(t) - 1, ax.st_lo, ax.st_hi);
+    v[u] = t - (float)st[u];
+  }
+  // energies ascend with (u, lane): the first and last lanes bound the rows
+  const int r_lo = readlane_i(st[0], 0);
+  const int n_row = readlane_i(st[EPL - 1], 63) + 4 - r_lo;
+  if (n_row > kRowsCap) return false;
+  wave_fence();  // earlier reads of the scratch are complete
+  const float* base = s_S + __mul24(r_lo, kSliceStride) + im;
+  for (int rr = lane; rr < n_row; rr += 64) {
+    const float* p = base + __mul24(rr, kSliceStride);
+    wr.U[rr] = w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * p[3];
+  }
+  wave_fence();
+  for (int rr = lane; rr + 3 < n_row; rr += 64) {
+    const float r0 = wr.U[rr], r1 = wr.U[rr + 1], r2 = wr.U[rr + 2], r3 = wr.U[rr + 3];
+    const float a = r1 - r0, b = r2 - r1, c = r3 - r2;
+    const float d2 = b - a, d3 = (c - b) - d2;
+    wr.C[rr] = make_float4(r0, a, 0.5f * d2, (1.f / 6.f) * d3);
+  }
+  wave_fence();
+#pragma unroll
+  for (int u = 0; u < EPL; ++u) {
+    const float4 k = wr.C[st[u] - r_lo];
+    const float x = v[u];
+    out[u] = fmaf(x, fmaf(x - 1.f, fmaf(x - 2.f, k.w, k.z), k.y), k.x);
+  }
+  return true;
+}
+
+
 // Stage the rows [r_lo, r_lo + n_rows) of the (logT, log g) bilinear blend of
 // the NSX grid into LDS (row stride kSliceStride, mu padding zeroed).
 __device__ inline void stage_nsx_rows(const DevNsx& N, int iT, int ig, float aT, float ag, int r_lo, int n_rows,
