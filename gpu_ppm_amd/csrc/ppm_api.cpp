@@ -540,15 +540,30 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
     const double scale = vmax > 0 ? std::ldexp(1.0, -std::ilogb(vmax)) : 1.0;
     std::vector<float> f(nh.data.size());
     for (size_t k = 0; k < f.size(); ++k) f[k] = (float)(nh.data[k] * scale);
-    if ((e = c->d_nsx.ensure(sizeof(float) * f.size())) || (e = c->d_mu.ensure(sizeof(double) * nh.mu.size())) ||
+    // mu nodes, then the inverse denominators of the cubic Lagrange weights
+    // of the stencil starting at each node (cpu/nsx.hpp:262-275)
+    const size_t nm = nh.mu.size();
+    std::vector<double> mu(nm * 5, 0.0);
+    std::copy(nh.mu.begin(), nh.mu.end(), mu.begin());
+    for (size_t i = 0; i + 3 < nm; ++i) {
+      const double* x = nh.mu.data() + i;
+      for (int k = 0; k < 4; ++k) {
+        double d = 1;
+        for (int m = 0; m < 4; ++m)
+          if (m != k) d *= x[k] - x[m];
+        mu[nm + 4 * i + k] = 1.0 / d;
+      }
+    }
+    if ((e = c->d_nsx.ensure(sizeof(float) * f.size())) || (e = c->d_mu.ensure(sizeof(double) * mu.size())) ||
         (e = hipMemcpy(c->d_nsx.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(c->d_mu.p, nh.mu.data(), sizeof(double) * nh.mu.size(), hipMemcpyHostToDevice))) {
+        (e = hipMemcpy(c->d_mu.p, mu.data(), sizeof(double) * mu.size(), hipMemcpyHostToDevice))) {
       ppm_close(c);
       return PPM_ERR_DEVICE;
     }
     DevNsx& N = c->N;
     N.data = (const float*)c->d_nsx.p;
     N.mu = (const double*)c->d_mu.p;
+    N.mu_den = N.mu + nm;
     N.nT = cfg.n_logT;
     N.ng = cfg.n_logg;
     N.nE = cfg.n_logEkT;

@@ -231,8 +231,8 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   // ---- items: each spot's rings split into chunks of similar cost
   double total_cost = 0;
   for (double v : cost) total_cost += v;
-  const size_t acc_per_item = (size_t)g.n_phase * n_e;
-  const size_t cap_items = std::max<size_t>(1, ((size_t)1536 << 20) / (acc_per_item * sizeof(double)));
+  const size_t acc_per_item = (size_t)g.n_phase * n_e * batch_acc_copies();
+  const size_t cap_items = std::max<size_t>(1, ((size_t)3072 << 20) / (acc_per_item * sizeof(double)));
   const double target = std::min<double>((double)cap_items / n_echunk, 6.0 * slots / n_echunk);
   std::vector<BatchItem> items;
   std::vector<double> item_cost;

@@ -57,6 +57,7 @@ struct BatchLaunch {
 
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
 int batch_threads();
+int batch_acc_copies();  // accumulator copies per item
 int batch_energies();
 int batched_prepare(int atmosphere, size_t lds);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);

@@ -22,16 +22,70 @@ namespace ppm {
 
 namespace {
 
-constexpr int kBW = 3;                  // wavefronts per workgroup
+constexpr int kBWE = 2;                 // wavefronts splitting the energies
+constexpr int kBWP = 2;                 // wavefronts splitting the covered positions
+constexpr int kBW = kBWE * kBWP;        // wavefronts per workgroup
 constexpr int kBThreads = 64 * kBW;
-constexpr int kBEPL = 2;                // energies per lane
-constexpr int kBEnergies = 64 * kBW * kBEPL;  // energies per workgroup (384)
+constexpr int kBEPL = 3;                // energies per lane
+constexpr int kBEnergies = 64 * kBWE * kBEPL;  // energies per workgroup (384)
+constexpr int kNP = 2;                  // patches evaluated together
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
+// pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096).
 struct EvRec {
-  int pos, bin;
-  double sw;
+  uint32_t pb;
+  float sw;
 };
+
+// Per-wavefront LDS of the sweep: the evaluation scratch and a queue of
+// covered positions (at most 63 pending + 64 new).
+constexpr int kQueueCap = 128;
+constexpr int kSweepBytes = kWaveScratchBytes + kQueueCap * 4;
+static_assert(kBEPL * 64 * 8 <= kSweepBytes, "segment totals are exchanged in the sweep scratch");
+
+// Position of covered position number c (0-based, ascending) in the coverage
+// bitmap, or n when fewer than c + 1 positions are covered.  Wave-uniform.
+__device__ inline int covered_position(const uint32_t* s_cov, int n_words, int c, int n, int lane) {
+  int before = 0;
+  for (int w0 = 0; w0 < n_words; w0 += 64) {
+    const int w = w0 + lane;
+    const uint32_t word = w < n_words ? s_cov[w] : 0u;
+    const int pc = __builtin_popcount(word);
+    int incl = pc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    const int chunk = __shfl(incl, 63);
+    if (before + chunk > c) {
+      const int L = __builtin_ctzll(__ballot(before + incl > c));
+      uint32_t wd = (uint32_t)__shfl((int)word, L);
+      for (int k = c - before - __shfl(incl - pc, L); k > 0; --k) wd &= wd - 1;
+      return (w0 + L) * 32 + __builtin_ctz(wd);
+    }
+    before += chunk;
+  }
+  return n;
+}
+
+// Number of covered positions.  Wave-uniform.
+__device__ inline int covered_count(const uint32_t* s_cov, int n_words, int lane) {
+  int c = 0;
+  for (int w = lane; w < n_words; w += 64) c += __builtin_popcount(s_cov[w]);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+  return c;
+}
+
+// Number of (sorted) events before position p.  Wave-uniform.
+__device__ inline int events_before(const EvRec* s_ev, int n_ev, int p, int lane) {
+  int c = 0;
+  for (int i = lane; i < n_ev; i += 64) c += (int)(s_ev[i].pb >> 12) < p;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+  return c;
+}
 
 struct BLds {
   size_t ph, mu, fI, zf, cs, S, cov, keys, sorted, total;
@@ -57,18 +111,12 @@ __host__ __device__ inline BLds batch_layout(int nf1, int nsx_rows, int cov_word
   l.cov = o;
   o += (size_t)cov_words * 4;
   o = (o + 15) & ~(size_t)15;
-  l.keys = o;
-  o += (size_t)ev_cap * 8;
+  l.keys = o;  // event keys; during the sweep, each wavefront's scratch and queue
+  o += max((size_t)ev_cap * 8, (size_t)kBW * kSweepBytes);
   l.sorted = o;
   o += (size_t)ev_cap * sizeof(EvRec);
   l.total = (o + 15) & ~(size_t)15;
   return l;
-}
-
-__device__ inline uint64_t rfl64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
@@ -79,8 +127,9 @@ __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
 }
 
 template <int ATM>
-__global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchLaunch B) {
+__global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, BatchLaunch B) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wq = wave % kBWE, wp = wave / kBWE;  // energy part, position part
   const int nf1 = B.n_fine + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rows_alloc = ATM == PPM_ATM_NSX ? B.nsx_rows : 0;
@@ -94,6 +143,8 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
   uint32_t* s_cov = reinterpret_cast<uint32_t*>(smem + lay.cov);
   uint64_t* s_keys = reinterpret_cast<uint64_t*>(smem + lay.keys);
   EvRec* s_ev = reinterpret_cast<EvRec*>(smem + lay.sorted);
+  float4* s_C = reinterpret_cast<float4*>(smem + lay.keys + wave * kSweepBytes);
+  int* s_q = reinterpret_cast<int*>(smem + lay.keys + wave * kSweepBytes + kWaveScratchBytes);
   __shared__ int s_item, s_nev;
 
   if (ATM == PPM_ATM_NSX)
@@ -111,20 +162,26 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
     const int q_idx = s_item;
     if (q_idx >= B.n_items) break;
     const BatchItem It = B.items[B.order[q_idx]];
-    double* acc = B.acc + (size_t)B.order[q_idx] * B.n_phase * n_e;
-    // energies of this thread: e = e0 + wave*128 + u*64 + lane (ascending grid)
+    // one accumulator copy per position part: each address has one writer
+    double* acc = B.acc + ((size_t)B.order[q_idx] * kBWP + wp) * B.n_phase * n_e;
+    // energies of this thread: e = e0 + (wq*kBEPL + u)*64 + lane (ascending grid)
     const int e_hi = min(It.e0 + kBEnergies, n_e) - 1;
     const double l_ref = log10(B.energies[It.e0]);
     float le[kBEPL], Ef[kBEPL];
     int eidx[kBEPL];
 #pragma unroll
     for (int u = 0; u < kBEPL; ++u) {
-      const int e = It.e0 + wave * 64 * kBEPL + u * 64 + lane;
+      const int e = It.e0 + (wq * kBEPL + u) * 64 + lane;
       eidx[u] = e <= e_hi ? e : -1;
       const double E = B.energies[e <= e_hi ? e : e_hi];
       Ef[u] = (float)E;
       le[u] = (float)((log10(E) - l_ref) * inv_h);
     }
+    const bool asc = energies_ascend<kBEPL>(le, lane);
+    unsigned int n_valid = 0;  // evaluations per position by this lane
+#pragma unroll
+    for (int u = 0; u < kBEPL; ++u) n_valid += eidx[u] >= 0;
+    const WaveEnergies we = wave_energies<kBEPL>(le, lane);
     for (int b = 0; b < B.n_phase; ++b)
 #pragma unroll
       for (int u = 0; u < kBEPL; ++u)
@@ -146,7 +203,7 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
           s_cs[k] = gf[2 * nf1 + k];
         }
       }
-      int r_lo = 0;
+      int r_lo = 0, n_rows = 0;
       EAxis ax{0.f, 0.f, 0, 0};
       if (ATM == PPM_ATM_NSX) {
         const double lkT = log10(S.kT_nsx);
@@ -154,7 +211,7 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
         const double t_max = (log10(B.energies[e_hi]) + log10(Dp->zf_max) - lkT - N.logE_min) * inv_h;
         r_lo = clampi((int)floor(t_min - 0.01) - 1, 0, N.nE - 4);
         const int r_hi = clampi((int)floor(t_max + 0.01) - 1, 0, N.nE - 4) + 3;
-        int n_rows = r_hi - r_lo + 1;
+        n_rows = r_hi - r_lo + 1;
         if (n_rows > rows_alloc) {
           if (tid == 0) atomicOr(&B.desc[task].flags, 2);
           n_rows = rows_alloc;
@@ -166,11 +223,13 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
         stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBThreads);
       }
       __syncthreads();
-      const double tx_base = (l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo;
+      const TxMap txm{(float)((l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo),
+                      (float)(inv_h * 0.30102999566398120)};
       const float inv_kT = (float)S.inv_kT;
       const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
       const bool point = T.ring == 0;
       const int n = point ? 1 : Dp->n_pix;
+      const double inv_n = 1.0 / n;
       const int n_runs = point ? 1 : Dp->n_runs;
       const ClassSet cset = B.csets[point ? 0 : n / 4];
       const double c_shift = point ? Dp->ps_shift : 1.0 - 0.25 * Dp->s_shift2;
@@ -229,25 +288,48 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
             int rank = 0;
             for (int j = 0; j < n_ev; ++j) rank += s_keys[j] < ki;
             const ClassPt cp = B.cpts[C.first_pt + g0 + (int)((ki >> 12) & 0xfffff)];
-            s_ev[rank] = EvRec{(int)(ki >> 32), cp.bin, (ki & 1u) ? cp.w : -cp.w};
+            s_ev[rank] = EvRec{(uint32_t)(ki >> 32) << 12 | (uint32_t)cp.bin, (float)((ki & 1u) ? cp.w : -cp.w)};
           }
           __syncthreads();
 
-          // ---- sweep over covered positions; each wavefront owns its energies
+          // ---- sweep.  The covered positions are split in kBWP segments of
+          // equal count, [seg_lo, seg_hi) for this wavefront; it takes the
+          // events in its segment with a prefix that starts at zero, and the
+          // sum of the earlier segments is added to those events afterwards.
+          int seg_lo = 0, seg_hi = 0x7fffffff;
+          if (kBWP > 1) {
+            const int n_cov = covered_count(s_cov, n_words, lane);
+            if (wp > 0) seg_lo = covered_position(s_cov, n_words, n_cov * wp / kBWP, n, lane);
+            if (wp < kBWP - 1) seg_hi = covered_position(s_cov, n_words, n_cov * (wp + 1) / kBWP, n, lane);
+          }
+          const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
+          const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
           double P[kBEPL];
 #pragma unroll
           for (int u = 0; u < kBEPL; ++u) P[u] = 0.0;
-          // the next event is held in scalar registers; contributions are
-          // posted as no-return FP64 atomics to the item's private
-          // accumulator (one writer per address: deterministic)
-          int ev = 0;
+          // 64 events at a time are held one per lane; the next one is read
+          // into scalar registers.  Contributions are posted as no-return
+          // FP64 atomics to the item's private accumulator (one writer per
+          // address: deterministic)
+          int ev = ev_begin, win = ev_begin - 64;
+          uint32_t v_pb = 0;
+          float v_sw = 0.f;
           int ev_pos = 0x7fffffff, ev_bin = 0;
           double ev_sw = 0.0;
           auto load_ev = [&]() {
-            if (ev < n_ev) {
-              ev_pos = __builtin_amdgcn_readfirstlane(s_ev[ev].pos);
-              ev_bin = __builtin_amdgcn_readfirstlane(s_ev[ev].bin);
-              ev_sw = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, s_ev[ev].sw)));
+            if (ev < ev_end) {
+              if (ev - win >= 64) {
+                win = ev;
+                if (ev + lane < n_ev) {
+                  const EvRec r = s_ev[ev + lane];
+                  v_pb = r.pb;
+                  v_sw = r.sw;
+                }
+              }
+              const uint32_t pb = (uint32_t)readlane_i((int)v_pb, ev - win);
+              ev_pos = (int)(pb >> 12);
+              ev_bin = (int)(pb & 4095u);
+              ev_sw = (double)readlane_f(v_sw, ev - win);
             } else {
               ev_pos = 0x7fffffff;
             }
@@ -260,49 +342,129 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
             load_ev();
           };
           load_ev();
-          for (int base = 0; base < n; base += 64) {
-            const uint32_t c_lo = __builtin_amdgcn_readfirstlane(s_cov[base >> 5]);
-            const uint32_t c_hi = __builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]);
-            const uint64_t cov = (uint64_t)c_lo | ((uint64_t)c_hi << 32);
-            if (cov == 0) continue;
-            const int k = base + lane;
+          // covered positions, in ascending order, 64 per batch: one patch
+          // set-up per lane, then every lane evaluates each visible patch
+          auto run_batch = [&](int cnt, int q_len) {
+            const int k = lane < cnt ? s_q[lane] : 0;
             PatchVals pv;
             pv.im = -1;
-            if ((cov >> lane) & 1ull) {
-              const double v = (k + c_o) / n + 1.0;
+            if (lane < cnt) {
+              const double v = (k + c_o) * inv_n + 1.0;
               double pp = v - floor(v);
               if (pp < s_ph[0]) pp += 1.;
-              patch_setup<ATM>(pp, 1.0, R, s_mu, N.nm, N.mu_min, tx_base, inv_h, pv);
+              patch_setup<ATM>(pp, 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
             }
-            uint64_t todo = cov;
+            // invisible patches add nothing to the prefix: the events between
+            // two visible positions fire with the same prefix
+            uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
             while (todo) {
-              const int q = __builtin_ctzll(todo);
-              todo &= todo - 1;
-              const int pos = base + q;
-              // window ends at or before pos use the prefix before F(pos)
-              while (ev_pos <= pos) fire();
-              const int im = readlane_i(pv.im, q);
-              if (im < 0) continue;
+              // up to kNP visible patches; a missing one repeats the first
+              int qs[kNP];
+#pragma unroll
+              for (int j = 0; j < kNP; ++j) {
+                qs[j] = todo ? __builtin_ctzll(todo) : -1;
+                todo &= todo - 1;
+              }
+              float val[kNP][kBEPL];
               if (ATM == PPM_ATM_NSX) {
-                const float tx = readlane_f(pv.tx, q);
-                const float w[4] = {readlane_f(pv.w[0], q), readlane_f(pv.w[1], q), readlane_f(pv.w[2], q),
-                                    readlane_f(pv.w[3], q)};
-                const float* rows = s_S + im;
+                float tx[kNP], w[kNP][4];
+                int im[kNP];
 #pragma unroll
-                for (int u = 0; u < kBEPL; ++u) P[u] += (double)nsx_eval(le[u] + tx, ax, rows, w);
-                n_done += (eidx[0] >= 0) + (eidx[1] >= 0);
-              } else {
-                const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
+                for (int j = 0; j < kNP; ++j) {
+                  const int q = qs[j] >= 0 ? qs[j] : qs[0];
+                  tx[j] = readlane_f(pv.tx, q);
+                  im[j] = readlane_i(pv.im, q);
 #pragma unroll
-                for (int u = 0; u < kBEPL; ++u) {
-                  const float Ee = Ef[u] * z;
-                  P[u] += (double)(c * (Ee * Ee * Ee / expm1f(Ee * inv_kT)));
+                  for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
                 }
-                n_done += (eidx[0] >= 0) + (eidx[1] >= 0);
+                if (!nsx_eval_fast<kBEPL, kNP>(le, we, tx, im, w, ax, n_rows, s_S, lane, val)) {
+#pragma unroll
+                  for (int j = 0; j < kNP; ++j) {
+                    if (qs[j] < 0) break;
+                    const float t1[1] = {tx[j]};
+                    const int m1[1] = {im[j]};
+                    const float w1[1][4] = {{w[j][0], w[j][1], w[j][2], w[j][3]}};
+                    float v1[1][kBEPL];
+                    if (!nsx_eval_fast<kBEPL, 1>(le, we, t1, m1, w1, ax, n_rows, s_S, lane, v1) &&
+                        !(asc && nsx_eval_wave<kBEPL>(le, tx[j], im[j], w[j], ax, s_S, s_C, lane, v1[0]))) {
+#pragma unroll
+                      for (int u = 0; u < kBEPL; ++u) v1[0][u] = nsx_eval(le[u] + tx[j], ax, s_S + im[j], w[j]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBEPL; ++u) val[j][u] = v1[0][u];
+                  }
+                }
+              } else {
+#pragma unroll
+                for (int j = 0; j < kNP; ++j) {
+                  const int q = qs[j] >= 0 ? qs[j] : qs[0];
+                  const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
+#pragma unroll
+                  for (int u = 0; u < kBEPL; ++u) {
+                    const float Ee = Ef[u] * z;
+                    val[j][u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
+                  }
+                }
+              }
+              // in position order: window ends at or before pos use the
+              // prefix before F(pos)
+#pragma unroll
+              for (int j = 0; j < kNP; ++j) {
+                if (qs[j] < 0) break;
+                const int pos = readlane_i(k, qs[j]);
+                while (ev_pos <= pos) fire();
+#pragma unroll
+                for (int u = 0; u < kBEPL; ++u) P[u] += (double)val[j][u];
+                n_done += n_valid;
               }
             }
+            // drop the batch from the queue
+            const int rest = q_len - cnt;
+            const int moved = lane < rest ? s_q[cnt + lane] : 0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) s_q[lane] = moved;
+            __builtin_amdgcn_wave_barrier();
+            return rest;
+          };
+          int q_len = 0;
+          const int sweep_end = min(n, seg_hi);
+          for (int base = seg_lo & ~63; base < sweep_end; base += 64) {
+            uint64_t cov = (uint64_t)__builtin_amdgcn_readfirstlane(s_cov[base >> 5]) |
+                           ((uint64_t)__builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]) << 32);
+            if (seg_lo > base) cov &= ~0ull << (seg_lo - base);
+            if (sweep_end - base < 64) cov &= (1ull << (sweep_end - base)) - 1;
+            const uint32_t c_lo = (uint32_t)cov, c_hi = (uint32_t)(cov >> 32);
+            if (cov == 0) continue;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(c_hi, __builtin_amdgcn_mbcnt_lo(c_lo, 0u));
+            const bool mine = lane < 32 ? (c_lo >> lane) & 1u : (c_hi >> (lane - 32)) & 1u;
+            if (mine) s_q[q_len + (int)below] = base + lane;
+            __builtin_amdgcn_wave_barrier();
+            q_len += __builtin_popcount(c_lo) + __builtin_popcount(c_hi);
+            if (q_len >= 64) q_len = run_batch(64, q_len);
           }
-          while (ev < n_ev) fire();  // window ends past the last covered position
+          while (q_len > 0) q_len = run_batch(min(q_len, 64), q_len);
+          while (ev < ev_end) fire();  // window ends past the last covered position
+          if (kBWP > 1) {
+            // segment totals through the sweep scratch, then the offset of
+            // this segment (earlier segments in order) on its events
+            double* s_T = reinterpret_cast<double*>(smem + lay.keys);
+            __syncthreads();
+            if (wp < kBWP - 1)
+#pragma unroll
+              for (int u = 0; u < kBEPL; ++u) s_T[(size_t)wave * (kSweepBytes / 8) + u * 64 + lane] = P[u];
+            __syncthreads();
+            if (wp > 0) {
+#pragma unroll
+              for (int u = 0; u < kBEPL; ++u) {
+                P[u] = 0.0;
+                for (int w = 0; w < wp; ++w) P[u] += s_T[(size_t)(w * kBWE + wq) * (kSweepBytes / 8) + u * 64 + lane];
+              }
+              ev = ev_begin;
+              win = ev_begin - 64;
+              load_ev();
+              while (ev < ev_end) fire();
+            }
+          }
         }
       }
     }
@@ -314,7 +476,7 @@ __global__ __launch_bounds__(kBThreads) void k_integral_batched(DevNsx N, BatchL
 }
 
 // flux[s][b][e] = sum over the star's items (in item order) of
-// scale(spot) * acc / E.
+// scale(spot) * (sum of the item's accumulator copies) / E.
 __global__ void k_reduce_batched(BatchLaunch B) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t per = (size_t)B.n_phase * B.n_e;
@@ -322,8 +484,11 @@ __global__ void k_reduce_batched(BatchLaunch B) {
   const int s = (int)(idx / per);
   const size_t o = idx - (size_t)s * per;
   double total = 0.0;
-  for (int it = B.star_item0[s]; it < B.star_item0[s + 1]; ++it)
-    total += B.spots[B.items[it].spot].scale * B.acc[(size_t)it * per + o];
+  for (int it = B.star_item0[s]; it < B.star_item0[s + 1]; ++it) {
+    double a = 0.0;
+    for (int c = 0; c < kBWP; ++c) a += B.acc[((size_t)it * kBWP + c) * per + o];
+    total += B.spots[B.items[it].spot].scale * a;
+  }
   B.flux[idx] = total / B.energies[o % B.n_e];
 }
 
@@ -334,6 +499,7 @@ size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, 
 }
 
 int batch_threads() { return kBThreads; }
+int batch_acc_copies() { return kBWP; }
 int batch_energies() { return kBEnergies; }
 
 int batched_prepare(int atmosphere, size_t lds) {

@@ -53,9 +53,16 @@ struct PatchVals {
   int im;         // first mu node, -1: skip
 };
 
+// Energy-axis offset of a patch: tx = tx_base + log10(zf) / h in slice rows,
+// taken through the FP32 logarithm (the evaluation itself runs in FP32).
+struct TxMap {
+  float base;   // tx_base
+  float scale;  // log10(2) / h
+};
+
 template <int ATM>
-__device__ inline void patch_setup(double pp, double weight, const RingTab& R, const double* s_mu, int nm,
-                                   double mu_min, double tx_base, double inv_h, PatchVals& o) {
+__device__ inline void patch_setup(double pp, double weight, const RingTab& R, const double* s_mu,
+                                   const double* mu_den, int nm, double mu_min, TxMap txm, PatchVals& o) {
   o.tx = 0.f;
   o.w[0] = o.w[1] = o.w[2] = o.w[3] = 0.f;
   o.zf = 0.f;
@@ -77,14 +84,16 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
       o.w[1] = (float)(c * am);
     } else {  // cubic in the middle
       im = clampi(im - 1, 0, nm - 4);
-      const double x0 = s_mu[im], x1 = s_mu[im + 1], x2 = s_mu[im + 2], x3 = s_mu[im + 3];
-      o.w[0] = (float)(c * ((mu - x1) * (mu - x2) * (mu - x3) / ((x0 - x1) * (x0 - x2) * (x0 - x3))));
-      o.w[1] = (float)(c * ((mu - x0) * (mu - x2) * (mu - x3) / ((x1 - x0) * (x1 - x2) * (x1 - x3))));
-      o.w[2] = (float)(c * ((mu - x0) * (mu - x1) * (mu - x3) / ((x2 - x0) * (x2 - x1) * (x2 - x3))));
-      o.w[3] = (float)(c * ((mu - x0) * (mu - x1) * (mu - x2) / ((x3 - x0) * (x3 - x1) * (x3 - x2))));
+      const double d0 = mu - s_mu[im], d1 = mu - s_mu[im + 1], d2 = mu - s_mu[im + 2], d3 = mu - s_mu[im + 3];
+      const double* den = mu_den + 4 * im;
+      const double p01 = d0 * d1, p23 = d2 * d3;
+      o.w[0] = (float)(c * (d1 * p23 * den[0]));
+      o.w[1] = (float)(c * (d0 * p23 * den[1]));
+      o.w[2] = (float)(c * (p01 * d3 * den[2]));
+      o.w[3] = (float)(c * (p01 * d2 * den[3]));
     }
     o.im = im;
-    o.tx = (float)(tx_base + log10(zz) * inv_h);
+    o.tx = fmaf(log2f((float)zz), txm.scale, txm.base);
   } else {
     o.w[0] = (float)c;
     o.im = 0;
@@ -129,31 +138,35 @@ __device__ inline float readlane_f(float v, int lane) {
 }
 __device__ inline int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-__device__ inline void wave_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Per-wavefront LDS scratch of nsx_eval_wave: one float4 per slice row.
+constexpr int kWaveScratchBytes = 64 * 16;
+
+// True when the lane's energies ascend with (u, lane) across the wavefront,
+// the order nsx_eval_wave relies on to bound the rows it touches.
+template <int EPL>
+__device__ inline bool energies_ascend(const float (&le)[EPL], int lane) {
+  bool ok = true;
+#pragma unroll
+  for (int u = 0; u < EPL; ++u) {
+    const float next = __shfl_down(le[u], 1);
+    if (lane < 63) ok = ok && le[u] <= next;
+    if (u + 1 < EPL) ok = ok && __shfl(le[u], 63) <= __shfl(le[u + 1 < EPL ? u + 1 : u], 0);
+  }
+  return __all(ok);
 }
 
-// Rows of the contracted slice one wavefront keeps per patch.
-constexpr int kRowsCap = 128;
-
-// Per-wavefront LDS scratch of the contracted evaluation.
-struct WaveRows {
-  float* U;    // [kRowsCap] slice rows contracted over the patch's mu stencil
-  float4* C;   // [kRowsCap] cubic coefficients (Newton form) starting at each row
-};
-
-// All energies of a wavefront at one patch (uniform: tx, im, w).  The mu
-// stencil is contracted once per row the wavefront touches, each row start
-// gets the Newton coefficients of the cubic through 4 consecutive rows, and
-// each energy is then one coefficient read and three FMAs:
-//   I(v) = r0 + v (a + (v-1) (d2/2 + (v-2) d3/6)),  v = t - st in [0, 3],
-// the same cubic Lagrange polynomial as cpu/nsx.hpp:159-167 on those rows.
-// Returns false (nothing written) when the rows exceed the scratch.
+// All energies of a wavefront at one patch (uniform tx, im, w).  The patch's
+// mu stencil is contracted once per slice row the wavefront touches (one row
+// per lane), each row start gets the Newton coefficients of the cubic
+// through 4 consecutive rows,
+//   I(v) = r0 + v (D1 + (v-1) (D2/2 + (v-2) D3/6)),  v = t - st in [0, 3],
+// which is the cubic Lagrange polynomial of cpu/nsx.hpp:159-167 over those
+// rows, and each energy reads one coefficient set.  Requires energies that
+// ascend with (u, lane) (energies_ascend); returns false, with nothing
+// written, when the rows exceed one per lane (the caller then uses nsx_eval).
 template <int EPL>
-__device__ inline bool nsx_eval_rows(const float (&le)[EPL], float tx, int im, const float (&w)[4], const EAxis& ax,
-                                     const float* s_S, const WaveRows& wr, int lane, float (&out)[EPL]) {
+__device__ inline bool nsx_eval_wave(const float (&le)[EPL], float tx, int im, const float (&w)[4], const EAxis& ax,
+                                     const float* s_S, float4* s_C, int lane, float (&out)[EPL]) {
   int st[EPL];
   float v[EPL];
 #pragma unroll
@@ -162,33 +175,109 @@ __device__ inline bool nsx_eval_rows(const float (&le)[EPL], float tx, int im, c
     st[u] = med3i((int)floorf(t) - 1, ax.st_lo, ax.st_hi);
     v[u] = t - (float)st[u];
   }
-  // energies ascend with (u, lane): the first and last lanes bound the rows
   const int r_lo = readlane_i(st[0], 0);
-  const int n_row = readlane_i(st[EPL - 1], 63) + 4 - r_lo;
-  if (n_row > kRowsCap) return false;
-  wave_fence();  // earlier reads of the scratch are complete
-  const float* base = s_S + __mul24(r_lo, kSliceStride) + im;
-  for (int rr = lane; rr < n_row; rr += 64) {
-    const float* p = base + __mul24(rr, kSliceStride);
-    wr.U[rr] = w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * p[3];
+  const int n_c = readlane_i(st[EPL - 1], 63) - r_lo + 1;  // coefficient sets needed
+  if (n_c > 61) return false;
+  float U = 0.f;
+  if (lane < n_c + 3) {
+    const float* p = s_S + __mul24(r_lo + lane, kSliceStride) + im;
+    U = w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * p[3];
   }
-  wave_fence();
-  for (int rr = lane; rr + 3 < n_row; rr += 64) {
-    const float r0 = wr.U[rr], r1 = wr.U[rr + 1], r2 = wr.U[rr + 2], r3 = wr.U[rr + 3];
-    const float a = r1 - r0, b = r2 - r1, c = r3 - r2;
-    const float d2 = b - a, d3 = (c - b) - d2;
-    wr.C[rr] = make_float4(r0, a, 0.5f * d2, (1.f / 6.f) * d3);
+  const float U1 = __shfl_down(U, 1), U2 = __shfl_down(U, 2), U3 = __shfl_down(U, 3);
+  if (lane < n_c) {
+    const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
+    const float d2 = d1b - d1, d3 = (d1c - d1b) - d2;
+    s_C[lane] = make_float4(U, d1, 0.5f * d2, (1.f / 6.f) * d3);
   }
-  wave_fence();
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int u = 0; u < EPL; ++u) {
-    const float4 k = wr.C[st[u] - r_lo];
+    const float4 k = s_C[med3i(st[u] - r_lo, 0, 63)];
     const float x = v[u];
     out[u] = fmaf(x, fmaf(x - 1.f, fmaf(x - 2.f, k.w, k.z), k.y), k.x);
   }
+  __builtin_amdgcn_wave_barrier();
   return true;
 }
 
+// Wave-uniform description of a wavefront's energies on the local axis.
+struct WaveEnergies {
+  float le_first, le_last;  // lane 0's first and lane 63's last energy
+  bool fast;                // ascending and spanning at most 58 rows: nsx_eval_fast applies
+};
+
+template <int EPL>
+__device__ inline WaveEnergies wave_energies(const float (&le)[EPL], int lane) {
+  WaveEnergies we;
+  we.le_first = readlane_f(le[0], 0);
+  we.le_last = readlane_f(le[EPL - 1], 63);
+  we.fast = energies_ascend<EPL>(le, lane) && we.le_last - we.le_first <= 58.f;
+  return we;
+}
+
+// Lane i receives lane i+1's value (DPP wave_shl:1); lane 63 receives 0.
+__device__ inline float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// Interior case of nsx_eval_wave for NP patches at once (independent
+// chains, for latency hiding).  When no energy of the wavefront reaches a
+// clamp of the axis or of the stencil start (checked on the first and last
+// energy, the evaluation being monotone in the energy), the stencil start is
+// floor(t) - 1 and x = t - floor(t) in [0, 1).  Lane l contracts slice row
+// floor(t_first) - 1 + l over the patch's mu stencil, takes the next three
+// rows from its neighbours and forms the power-basis coefficients of the
+// cubic through the four rows at the nodes x = -1, 0, 1, 2:
+//   I(x) = a0 + x (a1 + x (a2 + x a3))
+// (the polynomial of cpu/nsx.hpp:159-167); each energy then fetches the
+// coefficients of its row from the lane that holds them.  `n_rows` is the
+// number of staged slice rows.  Returns false, with nothing written, unless
+// every patch is interior.
+template <int EPL, int NP>
+__device__ inline bool nsx_eval_fast(const float (&le)[EPL], const WaveEnergies& we, const float (&tx)[NP],
+                                     const int (&im)[NP], const float (&w)[NP][4], const EAxis& ax, int n_rows,
+                                     const float* s_S, int lane, float (&out)[NP][EPL]) {
+  bool ok = we.fast;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const float tf = we.le_first + tx[j], tl = we.le_last + tx[j];
+    ok = ok && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
+  }
+  if (!ok) return false;
+  float a0[NP], a1[NP], a2[NP], a3[NP];
+  int i0[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    i0[j] = (int)floorf(we.le_first + tx[j]);
+    const int row = min(i0[j] - 1 + lane, n_rows - 1);
+    const float* p = s_S + __mul24(row, kSliceStride) + im[j];
+    const float U = w[j][0] * p[0] + w[j][1] * p[1] + w[j][2] * p[2] + w[j][3] * p[3];
+    const float U1 = lane_next(U);
+    const float U2 = lane_next(U1);
+    const float U3 = lane_next(U2);
+    const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
+    const float D2 = d1b - d1, D3 = (d1c - d1b) - D2;
+    a2[j] = 0.5f * D2;
+    a3[j] = (1.f / 6.f) * D3;
+    a1[j] = d1 + a2[j] - a3[j];
+    a0[j] = U1;
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int u = 0; u < EPL; ++u) {
+      const float t = le[u] + tx[j];
+      const float f = floorf(t);
+      const float x = t - f;
+      const int src = ((int)f - i0[j]) << 2;
+      const float c0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j])));
+      const float c1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j])));
+      const float c2 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j])));
+      const float c3 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j])));
+      out[j][u] = fmaf(x, fmaf(x, fmaf(x, c3, c2), c1), c0);
+    }
+  return true;
+}
 
 // Stage the rows [r_lo, r_lo + n_rows) of the (logT, log g) bilinear blend of
 // the NSX grid into LDS (row stride kSliceStride, mu padding zeroed).

@@ -82,6 +82,7 @@ struct DevLensing {
 struct DevNsx {
   const float* data;   // [nT][ng][nE][nm], scaled by `scale`
   const double* mu;    // [nm] decreasing
+  const double* mu_den;  // [nm][4] inverse Lagrange denominators of the stencil starting at each node
   int nT, ng, nE, nm;
   double logT_min, logT_max, logg_min, logg_max, logE_min, logE_max;
   double scale;        // data = 10^logInuT3 * scale

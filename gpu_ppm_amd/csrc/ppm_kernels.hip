@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
 
 // Byte offsets of the plain integral kernel's dynamic LDS.
 struct LdsLayout {
-  size_t ph, mu, fI, zf, cs, S, total;
+  size_t ph, mu, fI, zf, cs, S, C, total;
 };
 __host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
   LdsLayout l;
@@ -427,12 +427,15 @@ __host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
   o = (o + 15) & ~(size_t)15;
   l.S = o;
   o += (size_t)nsx_rows * kSliceStride * sizeof(float);
+  l.C = o;  // per-wavefront evaluation scratch
+  o += (size_t)kWaves * kWaveScratchBytes;
   l.total = (o + 15) & ~(size_t)15;
   return l;
 }
 
 constexpr int kEPL = 2;                     // energies per lane
 constexpr int kEPerWave = 64 * kEPL;        // energies per wavefront / workgroup
+constexpr int kNP = 4;                      // patches evaluated together
 
 // One workgroup per (ring, 128-energy chunk); its 4 wavefronts take the
 // phase bins round-robin.  Within a wavefront each lane sets up one patch in
@@ -457,6 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
   float* s_zf = reinterpret_cast<float*>(smem + lay.zf);
   float* s_cs = reinterpret_cast<float*>(smem + lay.cs);
   float* s_S = reinterpret_cast<float*>(smem + lay.S);
+  float4* s_C = reinterpret_cast<float4*>(smem + lay.C) + wave * (kWaveScratchBytes / 16);
 
   {
     const double* gph = P.d_ph + (size_t)task * nf1;
@@ -495,7 +499,8 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
     stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBlock);
   }
   __syncthreads();
-  const double tx_base = (l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo;
+  const TxMap txm{(float)((l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo),
+                  (float)(inv_h * 0.30102999566398120)};
 
   float le[kEPL], Ef[kEPL];
   bool ok[kEPL];
@@ -507,6 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
     Ef[u] = (float)E;
     le[u] = (float)((log10(E) - l_ref) * inv_h);
   }
+  const bool asc = energies_ascend<kEPL>(le, lane);
+  const WaveEnergies we = wave_energies<kEPL>(le, lane);
   const float inv_kT = (float)S.inv_kT;
   const int n_act = Dp->n_act, n_runs = Dp->n_runs, n_pix = Dp->n_pix, s2 = Dp->s_shift2;
   const bool point = T.ring == 0;
@@ -545,29 +552,66 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
           const double v = pt.p + shift + 1.;
           double pp = v - floor(v);
           if (pp < s_ph[0]) pp += 1.;
-          patch_setup<ATM>(pp, pt.w, R, s_mu, N.nm, N.mu_min, tx_base, inv_h, pv);
+          patch_setup<ATM>(pp, pt.w, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
         }
-        // ---- evaluation: every lane evaluates each patch at its energies
-        const int nk = min(64, n_act - base);
+        // ---- evaluation: every lane evaluates each visible patch at its
+        // energies, kNP patches at a time
         float bsum[kEPL];
 #pragma unroll
         for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
-        for (int q = 0; q < nk; ++q) {
-          const int im = readlane_i(pv.im, q);
-          if (im < 0) continue;
+        uint64_t todo = __ballot(pv.im >= 0);
+        while (todo) {
+          int qs[kNP];  // a missing patch repeats the first
+#pragma unroll
+          for (int j = 0; j < kNP; ++j) {
+            qs[j] = todo ? __builtin_ctzll(todo) : -1;
+            todo &= todo - 1;
+          }
           if (ATM == PPM_ATM_NSX) {
-            const float tx = readlane_f(pv.tx, q);
-            const float w[4] = {readlane_f(pv.w[0], q), readlane_f(pv.w[1], q), readlane_f(pv.w[2], q),
-                                readlane_f(pv.w[3], q)};
-            const float* rows = s_S + im;
+            float tx[kNP], w[kNP][4];
+            int im[kNP];
 #pragma unroll
-            for (int u = 0; u < kEPL; ++u) bsum[u] += nsx_eval(le[u] + tx, ax, rows, w);
+            for (int j = 0; j < kNP; ++j) {
+              const int q = qs[j] >= 0 ? qs[j] : qs[0];
+              tx[j] = readlane_f(pv.tx, q);
+              im[j] = readlane_i(pv.im, q);
+#pragma unroll
+              for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
+            }
+            float val[kNP][kEPL];
+            if (nsx_eval_fast<kEPL, kNP>(le, we, tx, im, w, ax, n_rows, s_S, lane, val)) {
+#pragma unroll
+              for (int j = 0; j < kNP; ++j)
+                if (qs[j] >= 0)
+#pragma unroll
+                  for (int u = 0; u < kEPL; ++u) bsum[u] += val[j][u];
+              continue;
+            }
+#pragma unroll
+            for (int j = 0; j < kNP; ++j) {
+              if (qs[j] < 0) break;
+              const float t1[1] = {tx[j]};
+              const int m1[1] = {im[j]};
+              const float w1[1][4] = {{w[j][0], w[j][1], w[j][2], w[j][3]}};
+              float v1[1][kEPL];
+              if (!nsx_eval_fast<kEPL, 1>(le, we, t1, m1, w1, ax, n_rows, s_S, lane, v1) &&
+                  !(asc && nsx_eval_wave<kEPL>(le, tx[j], im[j], w[j], ax, s_S, s_C, lane, v1[0]))) {
+#pragma unroll
+                for (int u = 0; u < kEPL; ++u) v1[0][u] = nsx_eval(le[u] + tx[j], ax, s_S + im[j], w[j]);
+              }
+#pragma unroll
+              for (int u = 0; u < kEPL; ++u) bsum[u] += v1[0][u];
+            }
           } else {
-            const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
 #pragma unroll
-            for (int u = 0; u < kEPL; ++u) {
-              const float Ee = Ef[u] * z;
-              bsum[u] = fmaf(c, Ee * Ee * Ee / expm1f(Ee * inv_kT), bsum[u]);
+            for (int j = 0; j < kNP; ++j) {
+              if (qs[j] < 0) break;
+              const float c = readlane_f(pv.w[0], qs[j]), z = readlane_f(pv.zf, qs[j]);
+#pragma unroll
+              for (int u = 0; u < kEPL; ++u) {
+                const float Ee = Ef[u] * z;
+                bsum[u] = fmaf(c, Ee * Ee * Ee / expm1f(Ee * inv_kT), bsum[u]);
+              }
             }
           }
         }
