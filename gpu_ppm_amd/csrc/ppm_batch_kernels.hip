@@ -357,41 +357,54 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             // invisible patches add nothing to the prefix: the events between
             // two visible positions fire with the same prefix
             uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
+            PatchRows pr{0, false, false};
+            if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
+            const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.interior);
+            // FP32 partial sums of at most one batch, folded into the FP64
+            // prefix before an event fires and at the end of the batch
+            float pf[kBEPL];
+#pragma unroll
+            for (int u = 0; u < kBEPL; ++u) pf[u] = 0.f;
+            auto fold = [&]() {
+#pragma unroll
+              for (int u = 0; u < kBEPL; ++u) {
+                P[u] += (double)pf[u];
+                pf[u] = 0.f;
+              }
+            };
             while (todo) {
               // up to kNP visible patches; a missing one repeats the first
               int qs[kNP];
+              bool all_interior = true;
 #pragma unroll
               for (int j = 0; j < kNP; ++j) {
                 qs[j] = todo ? __builtin_ctzll(todo) : -1;
                 todo &= todo - 1;
+                if (qs[j] >= 0) all_interior = all_interior && ((interior >> qs[j]) & 1);
               }
               float val[kNP][kBEPL];
               if (ATM == PPM_ATM_NSX) {
                 float tx[kNP], w[kNP][4];
-                int im[kNP];
+                int im[kNP], i0[kNP];
 #pragma unroll
                 for (int j = 0; j < kNP; ++j) {
                   const int q = qs[j] >= 0 ? qs[j] : qs[0];
                   tx[j] = readlane_f(pv.tx, q);
                   im[j] = readlane_i(pv.im, q);
+                  i0[j] = readlane_i(pr.i0, q);
 #pragma unroll
                   for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
                 }
-                if (!nsx_eval_fast<kBEPL, kNP>(le, we, tx, im, w, ax, n_rows, s_S, lane, val)) {
+                if (we.fast) {
+                  if (all_interior)
+                    nsx_eval_rows<kBEPL, kNP, false>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                  else
+                    nsx_eval_rows<kBEPL, kNP, true>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                } else {
 #pragma unroll
                   for (int j = 0; j < kNP; ++j) {
                     if (qs[j] < 0) break;
-                    const float t1[1] = {tx[j]};
-                    const int m1[1] = {im[j]};
-                    const float w1[1][4] = {{w[j][0], w[j][1], w[j][2], w[j][3]}};
-                    float v1[1][kBEPL];
-                    if (!nsx_eval_fast<kBEPL, 1>(le, we, t1, m1, w1, ax, n_rows, s_S, lane, v1) &&
-                        !(asc && nsx_eval_wave<kBEPL>(le, tx[j], im[j], w[j], ax, s_S, s_C, lane, v1[0]))) {
-#pragma unroll
-                      for (int u = 0; u < kBEPL; ++u) v1[0][u] = nsx_eval(le[u] + tx[j], ax, s_S + im[j], w[j]);
-                    }
-#pragma unroll
-                    for (int u = 0; u < kBEPL; ++u) val[j][u] = v1[0][u];
+                    nsx_eval_edge<kBEPL>(le, asc, tx[j], im[j], w[j], ax, s_S, s_C, lane, val[j]);
                   }
                 }
               } else {
@@ -412,12 +425,16 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
               for (int j = 0; j < kNP; ++j) {
                 if (qs[j] < 0) break;
                 const int pos = readlane_i(k, qs[j]);
-                while (ev_pos <= pos) fire();
+                if (ev_pos <= pos) {
+                  fold();
+                  while (ev_pos <= pos) fire();
+                }
 #pragma unroll
-                for (int u = 0; u < kBEPL; ++u) P[u] += (double)val[j][u];
+                for (int u = 0; u < kBEPL; ++u) pf[u] += val[j][u];
                 n_done += n_valid;
               }
             }
+            fold();
             // drop the batch from the queue
             const int rest = q_len - cnt;
             const int moved = lane < rest ? s_q[cnt + lane] : 0;
@@ -428,21 +445,26 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           };
           int q_len = 0;
           const int sweep_end = min(n, seg_hi);
-          for (int base = seg_lo & ~63; base < sweep_end; base += 64) {
-            uint64_t cov = (uint64_t)__builtin_amdgcn_readfirstlane(s_cov[base >> 5]) |
-                           ((uint64_t)__builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]) << 32);
-            if (seg_lo > base) cov &= ~0ull << (seg_lo - base);
-            if (sweep_end - base < 64) cov &= (1ull << (sweep_end - base)) - 1;
-            const uint32_t c_lo = (uint32_t)cov, c_hi = (uint32_t)(cov >> 32);
-            if (cov == 0) continue;
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(c_hi, __builtin_amdgcn_mbcnt_lo(c_lo, 0u));
-            const bool mine = lane < 32 ? (c_lo >> lane) & 1u : (c_hi >> (lane - 32)) & 1u;
-            if (mine) s_q[q_len + (int)below] = base + lane;
-            __builtin_amdgcn_wave_barrier();
-            q_len += __builtin_popcount(c_lo) + __builtin_popcount(c_hi);
-            if (q_len >= 64) q_len = run_batch(64, q_len);
+          for (int base = seg_lo & ~63;;) {
+            const bool more = base < sweep_end;
+            if (more) {  // queue the covered positions of the next 64
+              uint64_t cov = (uint64_t)__builtin_amdgcn_readfirstlane(s_cov[base >> 5]) |
+                             ((uint64_t)__builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]) << 32);
+              if (seg_lo > base) cov &= ~0ull << (seg_lo - base);
+              if (sweep_end - base < 64) cov &= (1ull << (sweep_end - base)) - 1;
+              const uint32_t c_lo = (uint32_t)cov, c_hi = (uint32_t)(cov >> 32);
+              const uint32_t below = __builtin_amdgcn_mbcnt_hi(c_hi, __builtin_amdgcn_mbcnt_lo(c_lo, 0u));
+              const bool mine = lane < 32 ? (c_lo >> lane) & 1u : (c_hi >> (lane - 32)) & 1u;
+              if (mine) s_q[q_len + (int)below] = base + lane;
+              __builtin_amdgcn_wave_barrier();
+              q_len += __builtin_popcount(c_lo) + __builtin_popcount(c_hi);
+              base += 64;
+            }
+            if (q_len >= 64 || (!more && q_len > 0))
+              q_len = run_batch(min(q_len, 64), q_len);
+            else if (!more)
+              break;
           }
-          while (q_len > 0) q_len = run_batch(min(q_len, 64), q_len);
           while (ev < ev_end) fire();  // window ends past the last covered position
           if (kBWP > 1) {
             // segment totals through the sweep scratch, then the offset of

@@ -217,41 +217,67 @@ __device__ inline WaveEnergies wave_energies(const float (&le)[EPL], int lane) {
 
 // Lane i receives lane i+1's value (DPP wave_shl:1); lane 63 receives 0.
 __device__ inline float lane_next(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
-// Interior case of nsx_eval_wave for NP patches at once (independent
-// chains, for latency hiding).  When no energy of the wavefront reaches a
-// clamp of the axis or of the stencil start (checked on the first and last
-// energy, the evaluation being monotone in the energy), the stencil start is
-// floor(t) - 1 and x = t - floor(t) in [0, 1).  Lane l contracts slice row
-// floor(t_first) - 1 + l over the patch's mu stencil, takes the next three
-// rows from its neighbours and forms the power-basis coefficients of the
-// cubic through the four rows at the nodes x = -1, 0, 1, 2:
+// Classification of one patch, made once in its set-up lane.  When the
+// wavefront's energies span few enough slice rows (WaveEnergies::fast) the
+// patch is evaluated by nsx_eval_rows: without clamps when no energy reaches
+// a clamp of the axis or of the stencil start (checked on the first and last
+// energy, the evaluation being monotone in the energy), with them otherwise.
+// i0 - 1 is the stencil start of the wavefront's first energy.
+struct PatchRows {
+  int i0;
+  bool interior;  // no clamps
+  bool rows;      // nsx_eval_rows applies (with or without clamps)
+};
+
+__device__ inline PatchRows patch_rows(float tx, const WaveEnergies& we, const EAxis& ax) {
+  const float tf = we.le_first + tx, tl = we.le_last + tx;
+  PatchRows r;
+  r.rows = we.fast;
+  r.interior = we.fast && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
+  const float tc = __builtin_amdgcn_fmed3f(tf, ax.t_lo, ax.t_hi);
+  r.i0 = med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1);
+  return r;
+}
+
+// floor(t) as an integer (one instruction).
+__device__ inline int floor_to_int(float t) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(t));
+  return r;
+}
+
+// Contracted evaluation of NP patches at once (independent chains, for
+// latency hiding); every patch must have PatchRows::rows, and with
+// CLAMP = false PatchRows::interior.  For energy t = le + tx (clamped to the
+// axis ends) the stencil start is st = floor(t) - 1 (clamped to the legal
+// starts) and x = t - (st + 1), in [0, 1) away from the clamps.  Lane l
+// contracts slice row i0 - 1 + l over the patch's mu stencil, takes the next
+// three rows from its neighbours and forms the power-basis coefficients of
+// the cubic through the four rows at the nodes x = -1, 0, 1, 2:
 //   I(x) = a0 + x (a1 + x (a2 + x a3))
-// (the polynomial of cpu/nsx.hpp:159-167); each energy then fetches the
-// coefficients of its row from the lane that holds them.  `n_rows` is the
-// number of staged slice rows.  Returns false, with nothing written, unless
-// every patch is interior.
-template <int EPL, int NP>
-__device__ inline bool nsx_eval_fast(const float (&le)[EPL], const WaveEnergies& we, const float (&tx)[NP],
-                                     const int (&im)[NP], const float (&w)[NP][4], const EAxis& ax, int n_rows,
+// (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256); each
+// energy then fetches the coefficients of its row from the lane that holds
+// them.  `n_rows` is the number of staged slice rows.
+template <int EPL, int NP, bool CLAMP>
+__device__ inline void nsx_eval_rows(const float (&le)[EPL], const float (&tx)[NP], const int (&im)[NP],
+                                     const int (&i0)[NP], const float (&w)[NP][4], const EAxis& ax, int n_rows,
                                      const float* s_S, int lane, float (&out)[NP][EPL]) {
-  bool ok = we.fast;
+  // all LDS reads of a stage are issued before their results are used
+  float p[NP][4];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    const float tf = we.le_first + tx[j], tl = we.le_last + tx[j];
-    ok = ok && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
-  }
-  if (!ok) return false;
-  float a0[NP], a1[NP], a2[NP], a3[NP];
-  int i0[NP];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    i0[j] = (int)floorf(we.le_first + tx[j]);
     const int row = min(i0[j] - 1 + lane, n_rows - 1);
-    const float* p = s_S + __mul24(row, kSliceStride) + im[j];
-    const float U = w[j][0] * p[0] + w[j][1] * p[1] + w[j][2] * p[2] + w[j][3] * p[3];
+    const float* r = s_S + __mul24(row, kSliceStride) + im[j];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) p[j][m] = r[m];
+  }
+  float a0[NP], a1[NP], a2[NP], a3[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const float U = w[j][0] * p[j][0] + w[j][1] * p[j][1] + w[j][2] * p[j][2] + w[j][3] * p[j][3];
     const float U1 = lane_next(U);
     const float U2 = lane_next(U1);
     const float U3 = lane_next(U2);
@@ -262,21 +288,47 @@ __device__ inline bool nsx_eval_fast(const float (&le)[EPL], const WaveEnergies&
     a1[j] = d1 + a2[j] - a3[j];
     a0[j] = U1;
   }
+  float x[NP][EPL], c[NP][EPL][4];
 #pragma unroll
   for (int j = 0; j < NP; ++j)
 #pragma unroll
     for (int u = 0; u < EPL; ++u) {
-      const float t = le[u] + tx[j];
-      const float f = floorf(t);
-      const float x = t - f;
-      const int src = ((int)f - i0[j]) << 2;
-      const float c0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j])));
-      const float c1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j])));
-      const float c2 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j])));
-      const float c3 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j])));
-      out[j][u] = fmaf(x, fmaf(x, fmaf(x, c3, c2), c1), c0);
+      int f1;
+      if (CLAMP) {
+        const float t = __builtin_amdgcn_fmed3f(le[u] + tx[j], ax.t_lo, ax.t_hi);
+        f1 = med3i(floor_to_int(t), ax.st_lo + 1, ax.st_hi + 1);
+        x[j][u] = t - (float)f1;
+      } else {
+        const float t = le[u] + tx[j];
+        f1 = floor_to_int(t);
+        x[j][u] = __builtin_amdgcn_fractf(t);
+      }
+      const int src = (f1 - i0[j]) << 2;
+      c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j])));
+      c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j])));
+      c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j])));
+      c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j])));
     }
-  return true;
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int u = 0; u < EPL; ++u)
+      out[j][u] = fmaf(x[j][u], fmaf(x[j][u], fmaf(x[j][u], c[j][u][3], c[j][u][2]), c[j][u][1]), c[j][u][0]);
+}
+
+// One patch whose wavefront energies span too many rows for nsx_eval_rows:
+// contracted evaluation with clamps (nsx_eval_wave) when they fit the
+// scratch, else per energy (nsx_eval).
+template <int EPL>
+__device__ inline void nsx_eval_edge(const float (&le)[EPL], bool asc, float tx, int im, const float (&w)[4],
+                                     const EAxis& ax, const float* s_S, float4* s_C, int lane, float (&out)[EPL]) {
+  if (!(asc && nsx_eval_wave<EPL>(le, tx, im, w, ax, s_S, s_C, lane, out))) {
+#pragma unroll
+    for (int u = 0; u < EPL; ++u) {
+      out[u] = nsx_eval(le[u] + tx, ax, s_S + im, w);
+      __builtin_amdgcn_sched_barrier(0);  // rare path: keep its register use low
+    }
+  }
 }
 
 // Stage the rows [r_lo, r_lo + n_rows) of the (logT, log g) bilinear blend of

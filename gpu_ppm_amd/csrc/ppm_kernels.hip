@@ -560,48 +560,48 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
 #pragma unroll
         for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
         uint64_t todo = __ballot(pv.im >= 0);
+        PatchRows pr{0, false, false};
+        if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
+        const uint64_t interior = __ballot(pv.im >= 0 && pr.interior);
         while (todo) {
           int qs[kNP];  // a missing patch repeats the first
+          bool all_interior = true;
 #pragma unroll
           for (int j = 0; j < kNP; ++j) {
             qs[j] = todo ? __builtin_ctzll(todo) : -1;
             todo &= todo - 1;
+            if (qs[j] >= 0) all_interior = all_interior && ((interior >> qs[j]) & 1);
           }
           if (ATM == PPM_ATM_NSX) {
             float tx[kNP], w[kNP][4];
-            int im[kNP];
+            int im[kNP], i0[kNP];
 #pragma unroll
             for (int j = 0; j < kNP; ++j) {
               const int q = qs[j] >= 0 ? qs[j] : qs[0];
               tx[j] = readlane_f(pv.tx, q);
               im[j] = readlane_i(pv.im, q);
+              i0[j] = readlane_i(pr.i0, q);
 #pragma unroll
               for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
             }
             float val[kNP][kEPL];
-            if (nsx_eval_fast<kEPL, kNP>(le, we, tx, im, w, ax, n_rows, s_S, lane, val)) {
+            if (we.fast) {
+              if (all_interior)
+                nsx_eval_rows<kEPL, kNP, false>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+              else
+                nsx_eval_rows<kEPL, kNP, true>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+            } else {
 #pragma unroll
-              for (int j = 0; j < kNP; ++j)
-                if (qs[j] >= 0)
-#pragma unroll
-                  for (int u = 0; u < kEPL; ++u) bsum[u] += val[j][u];
-              continue;
-            }
-#pragma unroll
-            for (int j = 0; j < kNP; ++j) {
-              if (qs[j] < 0) break;
-              const float t1[1] = {tx[j]};
-              const int m1[1] = {im[j]};
-              const float w1[1][4] = {{w[j][0], w[j][1], w[j][2], w[j][3]}};
-              float v1[1][kEPL];
-              if (!nsx_eval_fast<kEPL, 1>(le, we, t1, m1, w1, ax, n_rows, s_S, lane, v1) &&
-                  !(asc && nsx_eval_wave<kEPL>(le, tx[j], im[j], w[j], ax, s_S, s_C, lane, v1[0]))) {
-#pragma unroll
-                for (int u = 0; u < kEPL; ++u) v1[0][u] = nsx_eval(le[u] + tx[j], ax, s_S + im[j], w[j]);
+              for (int j = 0; j < kNP; ++j) {
+                if (qs[j] < 0) break;
+                nsx_eval_edge<kEPL>(le, asc, tx[j], im[j], w[j], ax, s_S, s_C, lane, val[j]);
               }
-#pragma unroll
-              for (int u = 0; u < kEPL; ++u) bsum[u] += v1[0][u];
             }
+#pragma unroll
+            for (int j = 0; j < kNP; ++j)
+              if (qs[j] >= 0)
+#pragma unroll
+                for (int u = 0; u < kEPL; ++u) bsum[u] += val[j][u];
           } else {
 #pragma unroll
             for (int j = 0; j < kNP; ++j) {
