@@ -26,7 +26,10 @@ using namespace ppm;
 namespace ppm {
 
 ppm_status fail(ppm_ctx* c, ppm_status st, const std::string& msg) {
-  if (c) c->err = msg;
+  if (c) {
+    std::lock_guard<std::mutex> lk(c->err_mu);  // the serving thread reports too
+    c->err = msg;
+  }
   return st;
 }
 

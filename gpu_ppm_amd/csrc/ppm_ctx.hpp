@@ -87,6 +87,7 @@ struct ppm_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_i0 = nullptr, ev_i1 = nullptr, ev_staging = nullptr;
   std::string err;
+  std::mutex err_mu;
   std::mutex exec_mu;  // one evaluation on the device at a time
 
   ppm::LensingHost lens;

@@ -1,14 +1,282 @@
-// Serving loop (continuous batching of single-profile requests).
+// Serving loop: continuous batching of single-profile requests.
+//
+// ppm_submit validates a request, copies its star, grid and energy grid,
+// charges its device memory against the context's budget and queues it.
+// A worker thread owned by the context takes, whenever the device is free,
+// every queued request that shares the oldest request's grid (up to
+// kMaxBatch) and evaluates them in one launch sequence (the batched
+// persistent work queue for two or more, the plain path for one), while
+// new requests keep arriving.  Results are copied to host memory of the
+// request and handed out by ppm_poll in any order.  The reference has no
+// serving layer; each request is one calc_one_star
+// (gpu/devarshi24st_gpu.cu:431-511) without the text output.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <unordered_map>
+
 #include "ppm_ctx.hpp"
 
 namespace ppm {
-void serve_init(ppm_ctx*) {}
-void serve_shutdown(ppm_ctx*) {}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kMaxBatch = 1024;          // requests per launch sequence
+constexpr size_t kMaxQueued = 1 << 16;   // queued + running requests
+constexpr size_t kDefaultBudget = (size_t)16 << 30;
+
+enum class ReqState { kQueued, kRunning, kDone, kFailed, kCancelled };
+
+struct Request {
+  ppm_star star{};
+  ppm_grid grid{};
+  std::vector<double> energies;  // the grid's energies, owned here
+  int n_ch = 0;                  // channels of the fold, 0 without
+  size_t bytes = 0;              // device memory charged at admission
+  std::vector<double> flux, counts;
+  Clock::time_point t_submit, t_done;
+  ReqState state = ReqState::kQueued;
+  bool cancel_pending = false;   // cancelled while running
+  ppm_status status = PPM_OK;
+};
+
+// Requests evaluate together when everything but the star is equal.
+bool same_grid(const Request& a, const Request& b) {
+  const ppm_grid &x = a.grid, &y = b.grid;
+  return x.n_side == y.n_side && x.n_phase == y.n_phase && x.sampling == y.sampling &&
+         (x.sampling != PPM_SAMPLE_GL || x.n_gl == y.n_gl) && x.n_fine == y.n_fine && x.response == y.response &&
+         a.energies == b.energies && a.star.atmosphere == b.star.atmosphere;
+}
+
+}  // namespace
+
+struct ServeState {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> queue;  // queued tickets, arrival order
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  uint64_t next_ticket = 1;
+  size_t budget = kDefaultBudget, charged = 0;
+  int active = 0;  // queued + running
+  bool stop = false;
+  std::thread worker;
+  // launch buffers of the worker
+  DevBuf d_flux, d_counts;
+  HostBuf h_flux, h_counts;
+};
+
+namespace {
+
+void worker_loop(ppm_ctx* c) {
+  ServeState& S = *c->serve;
+  std::vector<Request*> batch;
+  std::vector<uint64_t> tickets;
+  std::vector<ppm_star> stars;
+  for (;;) {
+    batch.clear();
+    tickets.clear();
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty(); });
+      if (S.stop) return;
+      // the oldest request and every queued request on the same grid
+      Request* first = S.reqs.at(S.queue.front()).get();
+      for (auto it = S.queue.begin(); it != S.queue.end() && (int)batch.size() < kMaxBatch;) {
+        Request* r = S.reqs.at(*it).get();
+        if (same_grid(*first, *r)) {
+          r->state = ReqState::kRunning;
+          batch.push_back(r);
+          tickets.push_back(*it);
+          it = S.queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    const int n = (int)batch.size();
+    const Request& g0 = *batch[0];
+    ppm_grid grid = g0.grid;
+    grid.energies_keV = g0.energies.data();
+    grid.n_energy = (int)g0.energies.size();
+    grid.method = n > 1 ? PPM_METHOD_BATCHED : PPM_METHOD_PLAIN;
+    stars.resize(n);
+    for (int k = 0; k < n; ++k) stars[k] = batch[k]->star;
+    const size_t per_flux = (size_t)grid.n_phase * grid.n_energy, per_counts = (size_t)g0.n_ch * grid.n_phase;
+    ppm_status st = PPM_OK;
+    {
+      std::lock_guard<std::mutex> lk(c->exec_mu);
+      hipSetDevice(c->device);
+      int e = 0;
+      if ((e = S.d_flux.ensure(sizeof(double) * per_flux * n)) || (e = S.h_flux.ensure(sizeof(double) * per_flux * n)) ||
+          (per_counts && ((e = S.d_counts.ensure(sizeof(double) * per_counts * n)) ||
+                          (e = S.h_counts.ensure(sizeof(double) * per_counts * n))))) {
+        st = hip_fail(c, e, "serving buffers");
+      } else {
+        ppm_stats stats{};
+        st = eval_any(c, stars.data(), n, &grid, (double*)S.d_flux.p, per_counts ? (double*)S.d_counts.p : nullptr,
+                      &stats);
+        if (st == PPM_OK) {
+          if ((e = hipMemcpyAsync(S.h_flux.p, S.d_flux.p, sizeof(double) * per_flux * n, hipMemcpyDeviceToHost,
+                                  c->stream)) ||
+              (per_counts && (e = hipMemcpyAsync(S.h_counts.p, S.d_counts.p, sizeof(double) * per_counts * n,
+                                                 hipMemcpyDeviceToHost, c->stream))) ||
+              (e = hipStreamSynchronize(c->stream)))
+            st = hip_fail(c, e, "serving read-back");
+        }
+      }
+    }
+    const Clock::time_point now = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      for (int k = 0; k < n; ++k) {
+        Request* r = batch[k];
+        S.charged -= r->bytes;
+        --S.active;
+        if (r->cancel_pending) {
+          S.reqs.erase(tickets[k]);
+          continue;
+        }
+        r->t_done = now;
+        r->status = st;
+        if (st == PPM_OK) {
+          const double* f = (const double*)S.h_flux.p + per_flux * k;
+          r->flux.assign(f, f + per_flux);
+          if (per_counts) {
+            const double* q = (const double*)S.h_counts.p + per_counts * k;
+            r->counts.assign(q, q + per_counts);
+          }
+          r->state = ReqState::kDone;
+        } else {
+          r->state = ReqState::kFailed;
+        }
+      }
+    }
+    S.cv_done.notify_all();
+  }
+}
+
+}  // namespace
+
+void serve_init(ppm_ctx* c) {
+  c->serve = new ServeState;
+  if (c->budget) c->serve->budget = c->budget;
+}
+
+void serve_shutdown(ppm_ctx* c) {
+  ServeState* S = c->serve;
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  S->d_flux.release();
+  S->d_counts.release();
+  S->h_flux.release();
+  S->h_counts.release();
+  delete S;
+  c->serve = nullptr;
+}
+
 }  // namespace ppm
 
+using namespace ppm;
+
 extern "C" {
-ppm_status ppm_submit(ppm_ctx*, const ppm_star*, const ppm_grid*, uint64_t*) { return PPM_ERR_UNSUPPORTED; }
-ppm_status ppm_poll(ppm_ctx*, uint64_t, double*, double*, double*) { return PPM_ERR_UNSUPPORTED; }
-ppm_status ppm_cancel(ppm_ctx*, uint64_t) { return PPM_ERR_UNSUPPORTED; }
-ppm_status ppm_drain(ppm_ctx*) { return PPM_ERR_UNSUPPORTED; }
+
+ppm_status ppm_submit(ppm_ctx* c, const ppm_star* star, const ppm_grid* grid, uint64_t* ticket) {
+  if (!c || !star || !grid || !ticket || !c->serve) return PPM_ERR_ARG;
+  auto r = std::make_unique<Request>();
+  r->star = *star;
+  r->grid = *grid;
+  ppm_status s = resolve_energies(c, *grid, r->energies);
+  if (s) return s;
+  if ((s = validate(c, r->star, r->grid, (int)r->energies.size()))) return s;
+  r->grid.energies_keV = nullptr;  // the copy in r->energies is used
+  r->grid.n_energy = (int)r->energies.size();
+  r->n_ch = grid->response >= 0 ? c->resp[grid->response].n_ch : 0;
+  // device memory of the request: its outputs, the batched accumulators
+  // (two FP64 copies per item, a few items per star) and ring tables
+  const size_t flux = sizeof(double) * (size_t)grid->n_phase * r->energies.size();
+  r->bytes = flux * 9 + sizeof(double) * (size_t)r->n_ch * grid->n_phase + ((size_t)1 << 20);
+  ServeState& S = *c->serve;
+  {
+    std::lock_guard<std::mutex> lk(S.mu);
+    if ((size_t)S.active >= kMaxQueued) return fail(c, PPM_ERR_BUSY, "serving queue full");
+    if (S.charged + r->bytes > S.budget) return fail(c, PPM_ERR_BUSY, "device-memory budget of the serving queue in use");
+    if (!S.worker.joinable()) S.worker = std::thread(worker_loop, c);
+    S.charged += r->bytes;
+    ++S.active;
+    r->t_submit = Clock::now();
+    *ticket = S.next_ticket++;
+    S.queue.push_back(*ticket);
+    S.reqs.emplace(*ticket, std::move(r));
+  }
+  S.cv_work.notify_one();
+  return PPM_OK;
 }
+
+ppm_status ppm_poll(ppm_ctx* c, uint64_t ticket, double* flux, double* counts, double* latency_ms) {
+  if (!c || !c->serve) return PPM_ERR_ARG;
+  ServeState& S = *c->serve;
+  std::lock_guard<std::mutex> lk(S.mu);
+  auto it = S.reqs.find(ticket);
+  if (it == S.reqs.end()) return PPM_ERR_NOT_FOUND;
+  Request& r = *it->second;
+  switch (r.state) {
+    case ReqState::kQueued:
+    case ReqState::kRunning:
+      return PPM_PENDING;
+    case ReqState::kCancelled:
+      S.reqs.erase(it);
+      return PPM_ERR_CANCELLED;
+    case ReqState::kFailed: {
+      const ppm_status st = r.status;
+      S.reqs.erase(it);
+      return st;
+    }
+    case ReqState::kDone:
+      break;
+  }
+  if (flux) std::copy(r.flux.begin(), r.flux.end(), flux);
+  if (counts && !r.counts.empty()) std::copy(r.counts.begin(), r.counts.end(), counts);
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+  S.reqs.erase(it);
+  return PPM_OK;
+}
+
+ppm_status ppm_cancel(ppm_ctx* c, uint64_t ticket) {
+  if (!c || !c->serve) return PPM_ERR_ARG;
+  ServeState& S = *c->serve;
+  {
+    std::lock_guard<std::mutex> lk(S.mu);
+    auto it = S.reqs.find(ticket);
+    if (it == S.reqs.end()) return PPM_ERR_NOT_FOUND;
+    Request& r = *it->second;
+    if (r.state == ReqState::kQueued) {
+      S.queue.erase(std::find(S.queue.begin(), S.queue.end(), ticket));
+      S.charged -= r.bytes;
+      --S.active;
+      r.state = ReqState::kCancelled;
+    } else if (r.state == ReqState::kRunning) {
+      r.cancel_pending = true;  // the result is dropped when the launch ends
+    } else {
+      S.reqs.erase(it);  // finished: the result is discarded
+    }
+  }
+  S.cv_done.notify_all();
+  return PPM_OK;
+}
+
+ppm_status ppm_drain(ppm_ctx* c) {
+  if (!c || !c->serve) return PPM_ERR_ARG;
+  ServeState& S = *c->serve;
+  std::unique_lock<std::mutex> lk(S.mu);
+  S.cv_done.wait(lk, [&] { return S.active == 0; });
+  return PPM_OK;
+}
+
+}  // extern "C"

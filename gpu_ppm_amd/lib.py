@@ -201,7 +201,7 @@ class Context:
         lat = c_double()
         st = load().ppm_poll(self._h, ticket, _dptr(flux), _dptr(cnt) if cnt is not None else None,
                              byref(lat))
-        if st not in (abi.OK, abi.PENDING):
+        if st not in (abi.OK, abi.PENDING, abi.ERR_CANCELLED, abi.ERR_NOT_FOUND):
             self._check(st)
         return st, flux, cnt, lat.value
 

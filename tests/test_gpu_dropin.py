@@ -1,0 +1,53 @@
+"""The drop-in programs run from a directory holding the reference-format
+inputs (tables/lensing/std, tables/ism_inst/std, ext/model_data/
+nsx_H_v200804.out), write the reference's output files, and their contents
+match the reference CPU programs' outputs on the same inputs
+(tests/golden/ref_cpu, made by cpu/os1_cpu.cpp and cpu/devarshi24st_cpu.cpp)
+to 1e-4 max relative error per bin above 1e-6 of the peak."""
+import gzip
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_binding import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(os.path.dirname(HERE), "gpu_ppm_amd", "_bin")
+GOLD = os.path.join(HERE, "golden", "ref_cpu")
+DEVARSHI = [(n, s) for s in ("std", "high") for n in ("Ring_Eq", "Ring_Polar", "Crescent_Eq", "Crescent_Polar")]
+
+
+def run(prog, cwd):
+    exe = os.path.join(BIN, prog)
+    assert os.path.exists(exe), "drop-in program not built: " + exe
+    print("case: drop-in", prog, flush=True)
+    r = subprocess.run([exe], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def test_os1_gpu(golden_root):
+    r = run("os1_gpu", golden_root)
+    assert r.stdout.count("Time taken:") == 10
+    for k in "abcdefghij":
+        got = np.loadtxt(os.path.join(golden_root, "output", "os1%s_gpu.txt" % k))
+        ref = np.loadtxt(os.path.join(GOLD, "os1%s_cpu.txt" % k))
+        assert got.shape == ref.shape == (128,)
+        assert rel_err(got, ref) <= TOL, (k, rel_err(got, ref))
+
+
+def test_devarshi24st_gpu(golden_root):
+    r = run("devarshi24st_gpu", golden_root)
+    assert r.stdout.count("Output written to file") == 9  # warm-up star included
+    for name, setting in DEVARSHI:
+        stem = os.path.join(golden_root, "output", "%s_%s" % (name, setting))
+        got = np.loadtxt(stem + "_counts_gpu.txt")
+        with gzip.open(os.path.join(GOLD, "%s_%s_counts_cpu.txt.gz" % (name, setting)), "rt") as f:
+            ref = np.loadtxt(f)
+        assert got.shape == ref.shape == (270, 32)
+        assert rel_err(got, ref) <= TOL, (name, setting, rel_err(got, ref))
+        assert float(open(stem + "_time_gpu.txt").read()) > 0
