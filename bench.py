@@ -16,6 +16,10 @@ Workloads (the configurations the headline metric is quoted on):
            The plain path is timed on the same stars, one profile per launch.
   plain    one star (0.25 rad cap at 20 deg, M 1.4, R_eq 12 km, 600 Hz,
            i 60 deg, kT 0.35 keV) at the same resolution, one profile per step.
+
+The batched workload is followed by the serving leg (bench_serving: p50/p99
+request latency and sustained evaluations under open-loop arrivals) and the
+CPU baseline leg (the FP64 oracle on a bounded sample).
 """
 import argparse
 import json
@@ -39,7 +43,7 @@ def log(*a):
 def data_root():
     from gpu_ppm_amd import fixtures
     root = os.path.join(ROOT, "tests", "_data", "bench")
-    fixtures.ensure_all(root, small_nsx=False, full_nsx=True, lensing=("std", "wide"), responses=("std",))
+    fixtures.ensure_all(root, small_nsx=False, full_nsx=True, lensing=("std", "wide"), responses=("std", "std128"))
     return root
 
 
@@ -139,6 +143,75 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars):
                 graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms)
 
 
+def bench_serving(root, duration_s, burst, seed=7):
+    """Continuous-batching serving loop under open-loop Poisson arrivals.
+
+    Requests: a seeded reflecting random walk over the batched priors (2 %
+    of each range per step), resolution drawn uniformly from N_side
+    {256, 1024, 2048} x {32, 64, 128} bins (2 GL nodes) x {128, 360}
+    energies (folded through the 300 x 128 or 300 x 360 response).  The
+    capacity is first measured on a burst of `burst` requests submitted at
+    once; the arrival rate is then 80 % of it for `duration_s` seconds.
+    Latency is submit -> result on the host, reported by ppm_poll."""
+    import numpy as np
+    from gpu_ppm_amd import abi
+    from gpu_ppm_amd import scenarios as S
+    from gpu_ppm_amd.lib import Context
+    ctx = Context(root, lensing="wide", responses=("std", "std128"))
+    rng = np.random.default_rng(seed)
+    combos = [(ns, nph, ne) for ns in (256, 1024, 2048) for nph in (32, 64, 128) for ne in (128, 360)]
+    grids = {c: abi.make_grid(c[0], c[1], sampling=abi.SAMPLE_GL, n_gl=2, response=0 if c[2] == 360 else 1)
+             for c in combos}
+    n_ch = ctx.response_info(0)[0]
+    bufs = {c: (np.zeros((c[1], c[2])), np.zeros((n_ch, c[1]))) for c in combos}
+    walk = S.PriorWalk(seed)
+
+    def draw(n):
+        return [(walk.next(), combos[rng.integers(len(combos))]) for _ in range(n)]
+
+    def run(reqs, arrivals):
+        pending, lat, i = {}, [], 0
+        t0 = time.perf_counter()
+        while i < len(reqs) or pending:
+            now = time.perf_counter() - t0
+            while i < len(reqs) and (arrivals is None or arrivals[i] <= now):
+                star, c = reqs[i]
+                pending[ctx.submit(star, grids[c])] = c
+                i += 1
+            for t in list(pending):
+                c = pending[t]
+                st, _, _, l = ctx.poll(t, grids[c], out=bufs[c])
+                if st == abi.PENDING:
+                    continue
+                if st != abi.OK:
+                    raise RuntimeError("serving request failed: status %d" % st)
+                lat.append(l)
+                del pending[t]
+            time.sleep(0.0005)
+        return time.perf_counter() - t0, np.array(lat)
+
+    run(draw(len(combos)), None)  # warm-up: every grid once
+    t_burst, _ = run(draw(burst), None)
+    capacity = burst / t_burst
+    rate = 0.8 * capacity
+    n_req = max(1, int(rate * duration_s))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = draw(n_req)
+    log("bench: serving %d requests at %.1f/s (capacity %.1f/s)" % (n_req, rate, capacity))
+    dt, lat = run(reqs, arrivals)
+    ref_evals = 0
+    for star, c in reqs:
+        ref_evals += ctx.count_active(star, grids[c]) * c[1] * 2 * c[2]
+    ctx.close()
+    return {"requests": n_req, "offered_per_s": rate, "burst_capacity_per_s": capacity,
+            "completed_per_s": n_req / dt, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "evals_per_s_ref": ref_evals / dt,
+            "workload": "prior random walk (2%% steps), N_side {256,1024,2048} x bins {32,64,128} x 2 GL "
+                        "x E {128,360}, open-loop Poisson arrivals at 80%% of burst capacity for %.0f s"
+                        % duration_s}
+
+
 def bench_plain(root, steps, warmup):
     from gpu_ppm_amd import abi, hip
     from gpu_ppm_amd import scenarios as S
@@ -164,6 +237,9 @@ def main():
     ap.add_argument("--stars", type=int, default=256, help="stars per batched step")
     ap.add_argument("--plain-stars", type=int, default=64, help="stars timed on the plain path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="duration of the open-loop serving leg (0 skips it)")
+    ap.add_argument("--serve-burst", type=int, default=192, help="requests in the capacity burst")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (profiles shard by star or by "
@@ -209,6 +285,8 @@ def main():
         out["batched"] = {"graph_ms_per_step": r["graph_ms"], "integral_ms": r["kernel_ms"],
                           "setup_reduce_fold_ms": r["setup_reduce_fold_ms"], "n_eval_ref": r["n_eval_ref"],
                           "dedupe_factor": r["n_eval_ref"] / max(r["n_eval"], 1)}
+    if args.workload == "batched" and args.serve_seconds > 0:
+        out["serving"] = bench_serving(root, args.serve_seconds, args.serve_burst)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(root, r["stars"], r["lensing"])
     print(json.dumps(out), flush=True)

@@ -73,9 +73,12 @@ struct ClassCache {
   bool uploaded = false;
 };
 
+// The launch sequence is captured into a graph the second time the same
+// configuration is launched in a row (a one-off configuration, as in
+// serving, is launched directly) and replayed while it repeats.
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  std::vector<unsigned char> key;
+  std::vector<unsigned char> key, last_direct;
 };
 
 struct BatchState {
@@ -368,7 +371,7 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
     return hipSuccess;
   };
   GraphCache& G = BS->graph;
-  const bool use_graph = !c->no_graph;
+  const bool use_graph = !c->no_graph && ((G.exec && G.key == key) || G.last_direct == key);
   if (use_graph) {
     if (!G.exec || G.key != key) {
       if (G.exec) {
@@ -395,8 +398,9 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
     if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");
     if ((e = hipGraphLaunch(G.exec, c->stream))) return hip_fail(c, e, "graph launch");
     if ((e = hipEventRecord(c->ev2, c->stream))) return hip_fail(c, e, "event");
-  } else if ((e = record(true))) {
-    return hip_fail(c, e, "batched launch");
+  } else {
+    if ((e = record(true))) return hip_fail(c, e, "batched launch");
+    G.last_direct = key;
   }
   if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
 

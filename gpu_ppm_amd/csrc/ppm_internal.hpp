@@ -22,7 +22,6 @@ constexpr double kBoltzKeVK = 8.617333262E-8;
 constexpr double kGMsunCgs = 1.32712440041279419E16;
 
 constexpr int kMaxRuns = 16;      // active-pixel runs per ring kept in a descriptor
-constexpr int kEChunk = 64;       // energies per wavefront (one per lane)
 constexpr int kSliceStride = 69;  // LDS row stride of the NSX slice (67 mu + 2 zero pad)
 
 // Per-(star, spot) parameters, all FP64, prepared on the host.

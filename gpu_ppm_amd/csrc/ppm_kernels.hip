@@ -5,10 +5,11 @@
 //                     prefix scan; FP64 fine-phase table of flux/I, redshift,
 //                     cos(sigma') and time-delayed phase; ring-skip rule, gap
 //                     fill, closure; NSX (logT, log g) bracket.
-//   k_integral_plain  one workgroup per (ring, 64-energy chunk): NSX slice
+//   k_integral_plain  one workgroup per (ring, 128-energy chunk): NSX slice
 //                     blended into LDS, per-patch set-up in FP64 by one lane
-//                     each, FP32 cubic-Lagrange evaluation with one energy per
-//                     lane, per-bin Kahan accumulation in registers.
+//                     each, FP32 cubic-Lagrange evaluation with two energies
+//                     per lane (rows contracted over the mu stencil once per
+//                     patch), per-bin Kahan accumulation in registers.
 //   k_reduce          fixed-order FP64 sum of the per-ring partials.
 //   k_fold            response fold on MFMA (split BF16, FP32 accumulate).
 #include <hip/hip_runtime.h>

@@ -191,13 +191,17 @@ class Context:
         self._check(load().ppm_submit(self._h, byref(star), byref(grid), byref(t)))
         return t.value
 
-    def poll(self, ticket, grid):
-        """(status, flux, counts, latency_ms); status PENDING until done."""
-        ne = self.n_energy(grid)
-        flux = np.zeros((grid.n_phase, ne))
-        cnt = None
-        if grid.response >= 0:
-            cnt = np.zeros((self.response_info(grid.response)[0], grid.n_phase))
+    def poll(self, ticket, grid, out=None):
+        """(status, flux, counts, latency_ms); status PENDING until done.
+        out: optional (flux, counts) arrays to fill instead of new ones."""
+        if out is not None:
+            flux, cnt = out
+        else:
+            ne = self.n_energy(grid)
+            flux = np.zeros((grid.n_phase, ne))
+            cnt = None
+            if grid.response >= 0:
+                cnt = np.zeros((self.response_info(grid.response)[0], grid.n_phase))
         lat = c_double()
         st = load().ppm_poll(self._h, ticket, _dptr(flux), _dptr(cnt) if cnt is not None else None,
                              byref(lat))

@@ -146,3 +146,41 @@ def sample_priors(n, seed=0, u_max=0.55, atmosphere=abi.ATM_NSX):
                                  inclination_rad=inc, distance_km=abi.KPC_KM, exposure_s=1e6,
                                  atmosphere=atmosphere, variant=abi.VARIANT_OS1))
     return out
+
+
+class PriorWalk:
+    """Seeded reflecting random walk over the batched-evaluation priors (the
+    serving workload): each step moves every parameter by a uniform amount
+    within +-`step` of its range, reflecting at the bounds; steps that would
+    leave the lensing table's u axis are redrawn."""
+
+    KEYS = ("mass", "radius", "spin", "inc", "colat", "rho", "kT")
+
+    def __init__(self, seed=0, step=0.02, u_max=0.55, atmosphere=abi.ATM_NSX):
+        self.rng = np.random.default_rng(seed)
+        self.step, self.u_max, self.atmosphere = step, u_max, atmosphere
+        while True:
+            x = {k: self.rng.uniform(*PRIORS[k]) for k in self.KEYS}
+            if polar_u(x["mass"], x["radius"], x["spin"]) <= u_max:
+                break
+        self.x = x
+
+    @staticmethod
+    def _reflect(v, lo, hi):
+        w = hi - lo
+        v = (v - lo) % (2 * w)
+        return lo + (v if v <= w else 2 * w - v)
+
+    def next(self):
+        while True:
+            y = {}
+            for k in self.KEYS:
+                lo, hi = PRIORS[k]
+                y[k] = self._reflect(self.x[k] + self.rng.uniform(-self.step, self.step) * (hi - lo), lo, hi)
+            if polar_u(y["mass"], y["radius"], y["spin"]) <= self.u_max:
+                break
+        self.x = y
+        spot = abi.make_spot(y["colat"], y["rho"], y["kT"], membership=abi.MEMBER_CAP)
+        return abi.make_star([spot], mass_msun=y["mass"], radius_eq_km=y["radius"], spin_hz=y["spin"],
+                             inclination_rad=y["inc"], distance_km=abi.KPC_KM, exposure_s=1e6,
+                             atmosphere=self.atmosphere, variant=abi.VARIANT_OS1)
