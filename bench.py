@@ -212,6 +212,33 @@ def bench_serving(root, duration_s, burst, seed=7):
                         % duration_s}
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of a known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    from gpu_ppm_amd import hip
+    n = mib << 20
+    src, dst = hip.DeviceArray((n,), "u1"), hip.DeviceArray((n,), "u1")
+    for _ in range(4):  # the first is a warm-up
+        hip.copy_device(dst, src, n)
+    hip.synchronize()
+    return {"calibration": "device-to-device copy", "bytes_per_copy": n, "copies": 4}
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default batched workload, from
+    the committed rocprofv3 counter summary (FETCH_SIZE / WRITE_SIZE passes
+    scaled by a copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_integral_batched_256stars_counters.md")
+    if not os.path.exists(path):
+        return None
+    key = "traffic_bytes_per_launch %s:" % kernel
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return float(line[len(key):])
+    return None
+
+
 def bench_plain(root, steps, warmup):
     from gpu_ppm_amd import abi, hip
     from gpu_ppm_amd import scenarios as S
@@ -240,11 +267,16 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="duration of the open-loop serving leg (0 skips it)")
     ap.add_argument("--serve-burst", type=int, default=192, help="requests in the capacity burst")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (profiles shard by star or by "
               "ring range with one small reduction); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
     root = data_root()
     log("bench: data ready; workload", args.workload)
     if args.workload == "batched":
@@ -274,7 +306,10 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": r["kernel"],
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": committed_traffic(r["kernel"]) if args.workload == "batched" and args.stars == 256
+                     else None,
+                     "kernel": r["kernel"],
                      "kernel_ms": r["kernel_ms"], "roof": "FP32 vector ALU (no packed FP32)",
                      "units": unit_note, "ref_equivalent_tflops": ref_eq},
     }

@@ -672,48 +672,62 @@ __device__ inline uint16_t bf16_rne(float f) {
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // counts[b][c][p] = scale[b] * sum_e rsp[c][e] * flux[b][p][e]; one wave per
-// 16 (channel) x 16 (phase) tile, K in steps of 16 energies.
+// (16-phase tile, star, group of kFoldTiles 16-channel tiles), K in steps of
+// 16 energies: the flux tile is read from HBM once per channel group and
+// split into BF16 hi/lo once per K step, the response (L2-resident) is
+// streamed per channel tile.
 // v_mfma_f32_16x16x16bf16_1k lane layout: A[m = l%16][k = 4(l/16)+i],
 // B[k = 4(l/16)+i][n = l%16], D[m = 4(l/16)+i][n = l%16].
+constexpr int kFoldTiles = 10;
+
 __global__ __launch_bounds__(64) void k_fold(const uint16_t* __restrict__ rsp_hi,
                                              const uint16_t* __restrict__ rsp_lo, int n_ch,
                                              int n_e, const double* __restrict__ flux, int n_phase,
                                              const double* __restrict__ scale,
                                              double* __restrict__ counts) {
-  const int c0 = blockIdx.x * 16, p0 = blockIdx.y * 16, b = blockIdx.z;
+  const int p0 = blockIdx.x * 16, b = blockIdx.y, t0 = blockIdx.z * kFoldTiles;
   const int l = threadIdx.x, row = l & 15, kq = (l >> 4) * 4;
-  const int ch = c0 + row, ph = p0 + row;
-  const double* fb = flux + (size_t)b * n_phase * n_e;
-  float4_t acc = {0.f, 0.f, 0.f, 0.f};
+  const int ph = p0 + row;
+  const int n_tiles = min(kFoldTiles, (n_ch + 15) / 16 - t0);
+  const double* fb = flux + ((size_t)b * n_phase + ph) * n_e;
+  float4_t acc[kFoldTiles];
+#pragma unroll
+  for (int t = 0; t < kFoldTiles; ++t) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < n_e; k0 += 16) {
-    short4_t ah, al, bh, bl;
+    short4_t bh, bl;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = k0 + kq + i;
-      uint16_t h = 0, lo = 0;
-      if (ch < n_ch && e < n_e) {
-        h = rsp_hi[(size_t)ch * n_e + e];
-        lo = rsp_lo[(size_t)ch * n_e + e];
-      }
-      ah[i] = (short)h;
-      al[i] = (short)lo;
-      float fv = 0.f;
-      if (ph < n_phase && e < n_e) fv = (float)fb[(size_t)ph * n_e + e];
+      const float fv = (ph < n_phase && e < n_e) ? (float)fb[e] : 0.f;
       const uint16_t fh = bf16_rne(fv);
-      const uint16_t fl = bf16_rne(fv - bf16_to_f(fh));
       bh[i] = (short)fh;
-      bl[i] = (short)fl;
+      bl[i] = (short)bf16_rne(fv - bf16_to_f(fh));
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < kFoldTiles; ++t) {
+      const int ch = (t0 + t) * 16 + row;  // tiles past n_ch read zeros
+      short4_t ah, al;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int e = k0 + kq + i;
+        const bool ok = t < n_tiles && ch < n_ch && e < n_e;
+        ah[i] = ok ? (short)rsp_hi[(size_t)ch * n_e + e] : (short)0;
+        al[i] = ok ? (short)rsp_lo[(size_t)ch * n_e + e] : (short)0;
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[t], 0, 0, 0);
+    }
   }
   const double sc = scale[b];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int c = c0 + kq + i, p = p0 + row;
-    if (c < n_ch && p < n_phase) counts[((size_t)b * n_ch + c) * n_phase + p] = sc * (double)acc[i];
-  }
+  for (int t = 0; t < kFoldTiles; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = (t0 + t) * 16 + kq + i, p = p0 + row;
+      if (t < n_tiles && c < n_ch && p < n_phase)
+        counts[((size_t)b * n_ch + c) * n_phase + p] = sc * (double)acc[t][i];
+    }
 }
 
 }  // namespace
@@ -771,7 +785,8 @@ int launch_reduce(const PlainLaunch& P, double* d_red, void* stream) {
 int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_e,
                 const double* d_flux, int n_batch, int n_phase, const double* d_scale,
                 double* d_counts, void* stream) {
-  dim3 grid((n_ch + 15) / 16, (n_phase + 15) / 16, n_batch);
+  const int n_tiles = (n_ch + 15) / 16;
+  dim3 grid((n_phase + 15) / 16, n_batch, (n_tiles + kFoldTiles - 1) / kFoldTiles);
   hipLaunchKernelGGL(k_fold, grid, dim3(64), 0, (hipStream_t)stream, rsp_hi, rsp_lo, n_ch, n_e,
                      d_flux, n_phase, d_scale, d_counts);
   return hipGetLastError();

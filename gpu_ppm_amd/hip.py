@@ -14,7 +14,7 @@ import numpy as np
 from . import lib as _ppm
 
 _hip = None
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def runtime():
@@ -47,6 +47,11 @@ def device_count():
 
 def synchronize():
     _check(runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy of nbytes between two DeviceArrays (blocking)."""
+    _check(runtime().hipMemcpy(c_void_p(dst.ptr), c_void_p(src.ptr), nbytes, D2D), "hipMemcpy")
 
 
 class DeviceArray:
