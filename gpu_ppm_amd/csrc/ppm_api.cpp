@@ -39,7 +39,7 @@ ppm_status hip_fail(ppm_ctx* c, int e, const char* what) {
 
 int DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return hipSuccess;
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
   size_t want = std::max(bytes, (size_t)256);
@@ -49,7 +49,7 @@ int DevBuf::ensure(size_t bytes) {
   return hipSuccess;
 }
 void DevBuf::release() {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
 }

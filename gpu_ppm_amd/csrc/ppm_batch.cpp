@@ -84,7 +84,7 @@ struct GraphCache {
 struct BatchState {
   ClassCache classes;
   GraphCache graph;
-  DevBuf d_csets, d_chdr, d_cpts, d_order, d_star_item0, d_acc;
+  DevBuf d_csets, d_chdr, d_cpts, d_order, d_star_item0, d_acc, d_seg;
   int n_cu = 0;
 };
 
@@ -116,7 +116,8 @@ void release_batch_state(ppm_ctx* c) {
   BatchState* S = c->batch_state;
   if (!S) return;
   if (S->graph.exec) hipGraphExecDestroy(S->graph.exec);
-  for (DevBuf* b : {&S->d_csets, &S->d_chdr, &S->d_cpts, &S->d_order, &S->d_star_item0, &S->d_acc}) b->release();
+  for (DevBuf* b : {&S->d_csets, &S->d_chdr, &S->d_cpts, &S->d_order, &S->d_star_item0, &S->d_acc, &S->d_seg})
+    b->release();
   delete S;
   c->batch_state = nullptr;
 }
@@ -236,7 +237,7 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   for (double v : cost) total_cost += v;
   const size_t acc_per_item = (size_t)g.n_phase * n_e * batch_acc_copies();
   const size_t cap_items = std::max<size_t>(1, ((size_t)3072 << 20) / (acc_per_item * sizeof(double)));
-  const double target = std::min<double>((double)cap_items / n_echunk, 6.0 * slots / n_echunk);
+  const double target = std::min<double>((double)cap_items / n_echunk, 12.0 * slots / n_echunk);
   std::vector<BatchItem> items;
   std::vector<double> item_cost;
   std::vector<int> star_item0(n + 1, 0);
@@ -282,6 +283,7 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
       (e = BS->d_order.ensure(sizeof(int) * std::max(n_items, 1))) ||
       (e = BS->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) || (e = W.evals.ensure(sizeof(unsigned long long))) ||
       (e = BS->d_acc.ensure(sizeof(double) * acc_per_item * std::max(n_items, 1))) ||
+      (e = BS->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
     return hip_fail(c, e, "device allocation");
   std::vector<double> fold_scale(n);
@@ -324,6 +326,7 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   B.queue = (int*)W.counter.p;
   B.n_evals = (unsigned long long*)W.evals.p;
   B.acc = (double*)BS->d_acc.p;
+  B.seg = (double*)BS->d_seg.p;
   B.flux = d_flux;
   B.n_items = n_items;
   B.n_stars = n;

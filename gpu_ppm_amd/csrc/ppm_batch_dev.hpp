@@ -49,7 +49,8 @@ struct BatchLaunch {
   const int* star_item0;   // [n_stars + 1] item ranges per star
   int* queue;
   unsigned long long* n_evals;  // (position, energy) evaluations performed
-  double* acc;             // [n_items][n_phase][n_e]
+  double* acc;             // [n_items][copies][n_phase][n_e]
+  double* seg;             // [n_wg][segment totals], workgroup scratch
   double* flux;            // [n_stars][n_phase][n_e]
   int n_items, n_stars, n_e, n_phase, n_fine;
   int nsx_rows, cov_words, ev_cap, atmosphere;
@@ -58,6 +59,7 @@ struct BatchLaunch {
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
+size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 int batch_energies();
 int batched_prepare(int atmosphere, size_t lds);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);

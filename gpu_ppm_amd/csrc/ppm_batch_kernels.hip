@@ -22,13 +22,13 @@ namespace ppm {
 
 namespace {
 
-constexpr int kBWE = 2;                 // wavefronts splitting the energies
-constexpr int kBWP = 2;                 // wavefronts splitting the covered positions
-constexpr int kBW = kBWE * kBWP;        // wavefronts per workgroup
+constexpr int kBW = 4;                  // wavefronts per workgroup, each a segment of the positions
+constexpr int kBWP = kBW;
 constexpr int kBThreads = 64 * kBW;
-constexpr int kBEPL = 3;                // energies per lane
-constexpr int kBEnergies = 64 * kBWE * kBEPL;  // energies per workgroup (384)
-constexpr int kNP = 2;                  // patches evaluated together
+constexpr int kBEPL = 6;                // energies per lane
+constexpr int kBSets = 2;               // row sets of a lane's energies (3 u each)
+constexpr int kBEnergies = 64 * kBEPL;  // energies per workgroup (384)
+constexpr int kNP = 1;                  // patches evaluated together
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
 // pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096).
@@ -41,7 +41,6 @@ struct EvRec {
 // covered positions (at most 63 pending + 64 new).
 constexpr int kQueueCap = 128;
 constexpr int kSweepBytes = kWaveScratchBytes + kQueueCap * 4;
-static_assert(kBEPL * 64 * 8 <= kSweepBytes, "segment totals are exchanged in the sweep scratch");
 
 // Position of covered position number c (0-based, ascending) in the coverage
 // bitmap, or n when fewer than c + 1 positions are covered.  Wave-uniform.
@@ -129,7 +128,7 @@ __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
 template <int ATM>
 __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, BatchLaunch B) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wq = wave % kBWE, wp = wave / kBWE;  // energy part, position part
+  const int wp = wave;  // position part
   const int nf1 = B.n_fine + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rows_alloc = ATM == PPM_ATM_NSX ? B.nsx_rows : 0;
@@ -162,30 +161,29 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
     const int q_idx = s_item;
     if (q_idx >= B.n_items) break;
     const BatchItem It = B.items[B.order[q_idx]];
-    // one accumulator copy per position part: each address has one writer
-    double* acc = B.acc + ((size_t)B.order[q_idx] * kBWP + wp) * B.n_phase * n_e;
-    // energies of this thread: e = e0 + (wq*kBEPL + u)*64 + lane (ascending grid)
-    const int e_hi = min(It.e0 + kBEnergies, n_e) - 1;
+    // one accumulator copy per position part: each address has one writer;
+    // acc points at the item's first energy
+    double* acc = B.acc + ((size_t)B.order[q_idx] * kBWP + wp) * B.n_phase * n_e + It.e0;
+    // energies of this thread: e = e0 + u*64 + lane (ascending grid), valid
+    // while u*64 + lane <= e_lim
+    const int e_hi = min(It.e0 + kBEnergies, n_e) - 1, e_lim = e_hi - It.e0;
     const double l_ref = log10(B.energies[It.e0]);
     float le[kBEPL], Ef[kBEPL];
-    int eidx[kBEPL];
 #pragma unroll
     for (int u = 0; u < kBEPL; ++u) {
-      const int e = It.e0 + (wq * kBEPL + u) * 64 + lane;
-      eidx[u] = e <= e_hi ? e : -1;
-      const double E = B.energies[e <= e_hi ? e : e_hi];
+      const double E = B.energies[It.e0 + min(u * 64 + lane, e_lim)];
       Ef[u] = (float)E;
       le[u] = (float)((log10(E) - l_ref) * inv_h);
     }
     const bool asc = energies_ascend<kBEPL>(le, lane);
     unsigned int n_valid = 0;  // evaluations per position by this lane
 #pragma unroll
-    for (int u = 0; u < kBEPL; ++u) n_valid += eidx[u] >= 0;
-    const WaveEnergies we = wave_energies<kBEPL>(le, lane);
+    for (int u = 0; u < kBEPL; ++u) n_valid += u * 64 + lane <= e_lim;
+    const WaveEnergies<kBSets> we = wave_energies<kBEPL, kBSets>(le, lane);
     for (int b = 0; b < B.n_phase; ++b)
 #pragma unroll
       for (int u = 0; u < kBEPL; ++u)
-        if (eidx[u] >= 0) acc[(size_t)b * n_e + eidx[u]] = 0.0;
+        if (u * 64 + lane <= e_lim) acc[(size_t)b * n_e + u * 64 + lane] = 0.0;
 
     for (int task = It.task0; task < It.task1; ++task) {
       const RingDesc* Dp = B.desc + task;
@@ -337,7 +335,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           auto fire = [&]() {
 #pragma unroll
             for (int u = 0; u < kBEPL; ++u)
-              if (eidx[u] >= 0) unsafeAtomicAdd(&acc[(size_t)ev_bin * n_e + eidx[u]], ev_sw * P[u]);
+              if (u * 64 + lane <= e_lim) unsafeAtomicAdd(acc + (size_t)ev_bin * n_e + u * 64 + lane, ev_sw * P[u]);
             ++ev;
             load_ev();
           };
@@ -357,7 +355,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             // invisible patches add nothing to the prefix: the events between
             // two visible positions fire with the same prefix
             uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
-            PatchRows pr{0, false, false};
+            PatchRows<kBSets> pr{{0, 0}, false};
             if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
             const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.interior);
             // FP32 partial sums of at most one batch, folded into the FP64
@@ -385,21 +383,22 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
               float val[kNP][kBEPL];
               if (ATM == PPM_ATM_NSX) {
                 float tx[kNP], w[kNP][4];
-                int im[kNP], i0[kNP];
+                int im[kNP], i0[kNP][kBSets];
 #pragma unroll
                 for (int j = 0; j < kNP; ++j) {
                   const int q = qs[j] >= 0 ? qs[j] : qs[0];
                   tx[j] = readlane_f(pv.tx, q);
                   im[j] = readlane_i(pv.im, q);
-                  i0[j] = readlane_i(pr.i0, q);
+#pragma unroll
+                  for (int t = 0; t < kBSets; ++t) i0[j][t] = readlane_i(pr.i0[t], q);
 #pragma unroll
                   for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
                 }
                 if (we.fast) {
                   if (all_interior)
-                    nsx_eval_rows<kBEPL, kNP, false>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                    nsx_eval_rows<kBEPL, kNP, false, kBSets>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
                   else
-                    nsx_eval_rows<kBEPL, kNP, true>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                    nsx_eval_rows<kBEPL, kNP, true, kBSets>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
                 } else {
 #pragma unroll
                   for (int j = 0; j < kNP; ++j) {
@@ -467,25 +466,25 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           }
           while (ev < ev_end) fire();  // window ends past the last covered position
           if (kBWP > 1) {
-            // segment totals through the sweep scratch, then the offset of
-            // this segment (earlier segments in order) on its events
-            double* s_T = reinterpret_cast<double*>(smem + lay.keys);
-            __syncthreads();
+            // segment totals through this workgroup's global scratch, then the
+            // offset of this segment (earlier segments in order) on its events
+            double* g_T = B.seg + (size_t)blockIdx.x * kBWP * kBEPL * 64;
             if (wp < kBWP - 1)
 #pragma unroll
-              for (int u = 0; u < kBEPL; ++u) s_T[(size_t)wave * (kSweepBytes / 8) + u * 64 + lane] = P[u];
+              for (int u = 0; u < kBEPL; ++u) g_T[((size_t)wp * kBEPL + u) * 64 + lane] = P[u];
             __syncthreads();
             if (wp > 0) {
 #pragma unroll
               for (int u = 0; u < kBEPL; ++u) {
                 P[u] = 0.0;
-                for (int w = 0; w < wp; ++w) P[u] += s_T[(size_t)(w * kBWE + wq) * (kSweepBytes / 8) + u * 64 + lane];
+                for (int w = 0; w < wp; ++w) P[u] += g_T[((size_t)w * kBEPL + u) * 64 + lane];
               }
               ev = ev_begin;
               win = ev_begin - 64;
               load_ev();
               while (ev < ev_end) fire();
             }
+            __syncthreads();  // the scratch is read before the next group writes it
           }
         }
       }
@@ -522,6 +521,7 @@ size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, 
 
 int batch_threads() { return kBThreads; }
 int batch_acc_copies() { return kBWP; }
+size_t batch_seg_doubles() { return (size_t)kBWP * kBEPL * 64; }
 int batch_energies() { return kBEnergies; }
 
 int batched_prepare(int atmosphere, size_t lds) {

@@ -200,46 +200,60 @@ __device__ inline bool nsx_eval_wave(const float (&le)[EPL], float tx, int im, c
   return true;
 }
 
-// Wave-uniform description of a wavefront's energies on the local axis.
+// Wave-uniform description of a wavefront's energies on the local axis.  A
+// lane's EPL energies (u = 0..EPL-1, energy u*64 + lane) are split into SETS
+// row sets of EPL/SETS consecutive u; each set spans its own window of slice
+// rows, contracted separately by nsx_eval_rows.
+template <int SETS>
 struct WaveEnergies {
-  float le_first, le_last;  // lane 0's first and lane 63's last energy
-  bool fast;                // ascending and spanning at most 58 rows: nsx_eval_fast applies
+  float le_first[SETS], le_last[SETS];  // lane 0's first and lane 63's last energy of each set
+  bool fast;  // ascending and every set spanning at most 58 rows: nsx_eval_rows applies
 };
 
-template <int EPL>
-__device__ inline WaveEnergies wave_energies(const float (&le)[EPL], int lane) {
-  WaveEnergies we;
-  we.le_first = readlane_f(le[0], 0);
-  we.le_last = readlane_f(le[EPL - 1], 63);
-  we.fast = energies_ascend<EPL>(le, lane) && we.le_last - we.le_first <= 58.f;
+template <int EPL, int SETS>
+__device__ inline WaveEnergies<SETS> wave_energies(const float (&le)[EPL], int lane) {
+  static_assert(EPL % SETS == 0, "row sets split the lane's energies evenly");
+  constexpr int G = EPL / SETS;
+  WaveEnergies<SETS> we;
+  we.fast = energies_ascend<EPL>(le, lane);
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
+    we.le_first[s] = readlane_f(le[s * G], 0);
+    we.le_last[s] = readlane_f(le[s * G + G - 1], 63);
+    we.fast = we.fast && we.le_last[s] - we.le_first[s] <= 58.f;
+  }
   return we;
+}
+
+// Classification of one patch, made once in its set-up lane.  When the
+// wavefront's energies allow nsx_eval_rows (WaveEnergies::fast) the patch is
+// evaluated there: without clamps when no energy reaches a clamp of the axis
+// or of the stencil start (checked on the first and last energy of each set,
+// the evaluation being monotone in the energy), with them otherwise.
+// i0[s] - 1 is the stencil start of the first energy of set s.
+template <int SETS>
+struct PatchRows {
+  int i0[SETS];
+  bool interior;  // no clamps
+};
+
+template <int SETS>
+__device__ inline PatchRows<SETS> patch_rows(float tx, const WaveEnergies<SETS>& we, const EAxis& ax) {
+  PatchRows<SETS> r;
+  r.interior = we.fast;
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
+    const float tf = we.le_first[s] + tx, tl = we.le_last[s] + tx;
+    r.interior = r.interior && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
+    const float tc = __builtin_amdgcn_fmed3f(tf, ax.t_lo, ax.t_hi);
+    r.i0[s] = med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1);
+  }
+  return r;
 }
 
 // Lane i receives lane i+1's value (DPP wave_shl:1); lane 63 receives 0.
 __device__ inline float lane_next(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-
-// Classification of one patch, made once in its set-up lane.  When the
-// wavefront's energies span few enough slice rows (WaveEnergies::fast) the
-// patch is evaluated by nsx_eval_rows: without clamps when no energy reaches
-// a clamp of the axis or of the stencil start (checked on the first and last
-// energy, the evaluation being monotone in the energy), with them otherwise.
-// i0 - 1 is the stencil start of the wavefront's first energy.
-struct PatchRows {
-  int i0;
-  bool interior;  // no clamps
-  bool rows;      // nsx_eval_rows applies (with or without clamps)
-};
-
-__device__ inline PatchRows patch_rows(float tx, const WaveEnergies& we, const EAxis& ax) {
-  const float tf = we.le_first + tx, tl = we.le_last + tx;
-  PatchRows r;
-  r.rows = we.fast;
-  r.interior = we.fast && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
-  const float tc = __builtin_amdgcn_fmed3f(tf, ax.t_lo, ax.t_hi);
-  r.i0 = med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1);
-  return r;
 }
 
 // floor(t) as an integer (one instruction).
@@ -250,49 +264,56 @@ __device__ inline int floor_to_int(float t) {
 }
 
 // Contracted evaluation of NP patches at once (independent chains, for
-// latency hiding); every patch must have PatchRows::rows, and with
-// CLAMP = false PatchRows::interior.  For energy t = le + tx (clamped to the
-// axis ends) the stencil start is st = floor(t) - 1 (clamped to the legal
-// starts) and x = t - (st + 1), in [0, 1) away from the clamps.  Lane l
-// contracts slice row i0 - 1 + l over the patch's mu stencil, takes the next
-// three rows from its neighbours and forms the power-basis coefficients of
-// the cubic through the four rows at the nodes x = -1, 0, 1, 2:
+// latency hiding), every patch of a wavefront with WaveEnergies::fast and,
+// with CLAMP = false, PatchRows::interior.  For energy t = le + tx (clamped
+// to the axis ends) the stencil start is st = floor(t) - 1 (clamped to the
+// legal starts) and x = t - (st + 1), in [0, 1) away from the clamps.  For
+// each row set, lane l contracts slice row i0 - 1 + l over the patch's mu
+// stencil, takes the next three rows from its neighbours and forms the
+// power-basis coefficients of the cubic through the four rows at the nodes
+// x = -1, 0, 1, 2:
 //   I(x) = a0 + x (a1 + x (a2 + x a3))
 // (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256); each
 // energy then fetches the coefficients of its row from the lane that holds
 // them.  `n_rows` is the number of staged slice rows.
-template <int EPL, int NP, bool CLAMP>
+template <int EPL, int NP, bool CLAMP, int SETS>
 __device__ inline void nsx_eval_rows(const float (&le)[EPL], const float (&tx)[NP], const int (&im)[NP],
-                                     const int (&i0)[NP], const float (&w)[NP][4], const EAxis& ax, int n_rows,
-                                     const float* s_S, int lane, float (&out)[NP][EPL]) {
+                                     const int (&i0)[NP][SETS], const float (&w)[NP][4], const EAxis& ax,
+                                     int n_rows, const float* s_S, int lane, float (&out)[NP][EPL]) {
+  constexpr int G = EPL / SETS;
   // all LDS reads of a stage are issued before their results are used
-  float p[NP][4];
+  float p[NP][SETS][4];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    const int row = min(i0[j] - 1 + lane, n_rows - 1);
-    const float* r = s_S + __mul24(row, kSliceStride) + im[j];
+  for (int j = 0; j < NP; ++j)
 #pragma unroll
-    for (int m = 0; m < 4; ++m) p[j][m] = r[m];
-  }
-  float a0[NP], a1[NP], a2[NP], a3[NP];
+    for (int s = 0; s < SETS; ++s) {
+      const int row = min(i0[j][s] - 1 + lane, n_rows - 1);
+      const float* r = s_S + __mul24(row, kSliceStride) + im[j];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    const float U = w[j][0] * p[j][0] + w[j][1] * p[j][1] + w[j][2] * p[j][2] + w[j][3] * p[j][3];
-    const float U1 = lane_next(U);
-    const float U2 = lane_next(U1);
-    const float U3 = lane_next(U2);
-    const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
-    const float D2 = d1b - d1, D3 = (d1c - d1b) - D2;
-    a2[j] = 0.5f * D2;
-    a3[j] = (1.f / 6.f) * D3;
-    a1[j] = d1 + a2[j] - a3[j];
-    a0[j] = U1;
-  }
+      for (int m = 0; m < 4; ++m) p[j][s][m] = r[m];
+    }
+  float a0[NP][SETS], a1[NP][SETS], a2[NP][SETS], a3[NP][SETS];
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int s = 0; s < SETS; ++s) {
+      const float U = w[j][0] * p[j][s][0] + w[j][1] * p[j][s][1] + w[j][2] * p[j][s][2] + w[j][3] * p[j][s][3];
+      const float U1 = lane_next(U);
+      const float U2 = lane_next(U1);
+      const float U3 = lane_next(U2);
+      const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
+      const float D2 = d1b - d1, D3 = (d1c - d1b) - D2;
+      a2[j][s] = 0.5f * D2;
+      a3[j][s] = (1.f / 6.f) * D3;
+      a1[j][s] = d1 + a2[j][s] - a3[j][s];
+      a0[j][s] = U1;
+    }
   float x[NP][EPL], c[NP][EPL][4];
 #pragma unroll
   for (int j = 0; j < NP; ++j)
 #pragma unroll
     for (int u = 0; u < EPL; ++u) {
+      const int s = u / G;
       int f1;
       if (CLAMP) {
         const float t = __builtin_amdgcn_fmed3f(le[u] + tx[j], ax.t_lo, ax.t_hi);
@@ -303,11 +324,11 @@ __device__ inline void nsx_eval_rows(const float (&le)[EPL], const float (&tx)[N
         f1 = floor_to_int(t);
         x[j][u] = __builtin_amdgcn_fractf(t);
       }
-      const int src = (f1 - i0[j]) << 2;
-      c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j])));
-      c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j])));
-      c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j])));
-      c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j])));
+      const int src = (f1 - i0[j][s]) << 2;
+      c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j][s])));
+      c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j][s])));
+      c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j][s])));
+      c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j][s])));
     }
 #pragma unroll
   for (int j = 0; j < NP; ++j)

@@ -141,7 +141,6 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
   __shared__ int s_int[kWaves * 2];
   __shared__ double s_dbl[kWaves * 2];
   __shared__ int s_base[2];
-  __shared__ int s_nact;
 
   double z, d_omega;
   int n_pix, s2;
@@ -161,7 +160,6 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
     D->s_shift2 = s2;
     D->ps_shift = S.ps_shift;
     D->flags = 0;
-    s_nact = 0;
     s_base[0] = s_base[1] = 0;
   }
   __syncthreads();
@@ -514,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
     le[u] = (float)((log10(E) - l_ref) * inv_h);
   }
   const bool asc = energies_ascend<kEPL>(le, lane);
-  const WaveEnergies we = wave_energies<kEPL>(le, lane);
+  const WaveEnergies<1> we = wave_energies<kEPL, 1>(le, lane);
   const float inv_kT = (float)S.inv_kT;
   const int n_act = Dp->n_act, n_runs = Dp->n_runs, n_pix = Dp->n_pix, s2 = Dp->s_shift2;
   const bool point = T.ring == 0;
@@ -561,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
 #pragma unroll
         for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
         uint64_t todo = __ballot(pv.im >= 0);
-        PatchRows pr{0, false, false};
+        PatchRows<1> pr{{0}, false};
         if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
         const uint64_t interior = __ballot(pv.im >= 0 && pr.interior);
         while (todo) {
@@ -575,22 +573,22 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
           }
           if (ATM == PPM_ATM_NSX) {
             float tx[kNP], w[kNP][4];
-            int im[kNP], i0[kNP];
+            int im[kNP], i0[kNP][1];
 #pragma unroll
             for (int j = 0; j < kNP; ++j) {
               const int q = qs[j] >= 0 ? qs[j] : qs[0];
               tx[j] = readlane_f(pv.tx, q);
               im[j] = readlane_i(pv.im, q);
-              i0[j] = readlane_i(pr.i0, q);
+              i0[j][0] = readlane_i(pr.i0[0], q);
 #pragma unroll
               for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
             }
             float val[kNP][kEPL];
             if (we.fast) {
               if (all_interior)
-                nsx_eval_rows<kEPL, kNP, false>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, false, 1>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
               else
-                nsx_eval_rows<kEPL, kNP, true>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, true, 1>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
             } else {
 #pragma unroll
               for (int j = 0; j < kNP; ++j) {
