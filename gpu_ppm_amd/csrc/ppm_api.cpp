@@ -303,6 +303,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   }
   const int n_tasks = (int)tasks.size();
   int nsx_rows = st.atmosphere == PPM_ATM_NSX ? (rows_override ? rows_override : nsx_rows_for(c, energies)) : 0;
+  if (nsx_rows) nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
   if (integral_lds_bytes(g.n_fine, st.atmosphere, nsx_rows) > (size_t)c->max_lds)
     return fail(c, PPM_ERR_ARG, "n_fine too large for the LDS of one workgroup");
   Workspace& W = c->ws;

@@ -222,6 +222,7 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
       }
       nsx_rows = std::min(c->N.nE, (int)std::ceil(span) + 16 + 8);
     }
+    nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
   }
   const int cov_words = (4 * g.n_side + 63) / 64 * 2 + 2;
   const int ev_cap = 512;

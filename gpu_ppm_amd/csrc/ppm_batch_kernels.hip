@@ -28,7 +28,6 @@ constexpr int kBThreads = 64 * kBW;
 constexpr int kBEPL = 6;                // energies per lane
 constexpr int kBSets = 2;               // row sets of a lane's energies (3 u each)
 constexpr int kBEnergies = 64 * kBEPL;  // energies per workgroup (384)
-constexpr int kNP = 1;                  // patches evaluated together
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
 // pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096).
@@ -202,7 +201,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         }
       }
       int r_lo = 0, n_rows = 0;
-      EAxis ax{0.f, 0.f, 0, 0};
+      EAxis ax{0.f, 0.f, 0, 0, 1};
       if (ATM == PPM_ATM_NSX) {
         const double lkT = log10(S.kT_nsx);
         const double t_min = (l_ref + log10(Dp->zf_min) - lkT - N.logE_min) * inv_h;
@@ -218,6 +217,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         ax.t_hi = (float)(N.nE - 1 - r_lo);
         ax.st_lo = 0;
         ax.st_hi = min(N.nE - 4 - r_lo, n_rows - 4);
+        ax.i0_max = rows_alloc - 63;  // rows_alloc >= 64
         stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBThreads);
       }
       __syncthreads();
@@ -236,6 +236,13 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       for (int ci = 0; ci < cset.n_classes; ++ci) {
         const ClassHdr C = B.chdr[cset.first_class + ci];
         const double c_o = C.delta + c_shift;
+        // rotational phase of position k, wrapped into [ph[0], ph[0] + 1)
+        auto phase_of = [&](int k) {
+          const double v = (k + c_o) * inv_n + 1.0;
+          double pp = v - floor(v);
+          if (pp < s_ph[0]) pp += 1.;
+          return pp;
+        };
         const int per_group = max(1, B.ev_cap / (4 * n_runs));
         for (int g0 = 0; g0 < C.n_pts; g0 += per_group) {
           const int g1 = min(C.n_pts, g0 + per_group);
@@ -287,6 +294,22 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             for (int j = 0; j < n_ev; ++j) rank += s_keys[j] < ki;
             const ClassPt cp = B.cpts[C.first_pt + g0 + (int)((ki >> 12) & 0xfffff)];
             s_ev[rank] = EvRec{(uint32_t)(ki >> 32) << 12 | (uint32_t)cp.bin, (float)((ki & 1u) ? cp.w : -cp.w)};
+          }
+          // ---- keep the covered positions whose patch is seen (flux/I != 0):
+          // the others add nothing, so the split, the queue and the set-up
+          // only count patches that contribute.  Each wavefront rewrites its
+          // own 64-position chunks.
+          for (int base = wave * 64; base < n; base += kBThreads) {
+            const int k = base + lane;
+            bool keep = false;
+            if (k < n && ((s_cov[k >> 5] >> (k & 31)) & 1u)) {
+              int fi;
+              double fa;
+              keep = patch_flux_factor(phase_of(k), R, fi, fa) != 0;
+            }
+            const uint64_t m = __ballot(keep);
+            if (lane == 0) s_cov[base >> 5] = (uint32_t)m;
+            if (lane == 32) s_cov[(base >> 5) + 1] = (uint32_t)(m >> 32);
           }
           __syncthreads();
 
@@ -346,17 +369,12 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             const int k = lane < cnt ? s_q[lane] : 0;
             PatchVals pv;
             pv.im = -1;
-            if (lane < cnt) {
-              const double v = (k + c_o) * inv_n + 1.0;
-              double pp = v - floor(v);
-              if (pp < s_ph[0]) pp += 1.;
-              patch_setup<ATM>(pp, 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
-            }
-            // invisible patches add nothing to the prefix: the events between
-            // two visible positions fire with the same prefix
+            if (lane < cnt) patch_setup<ATM>(phase_of(k), 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
+            // skipped patches add nothing to the prefix: the events between
+            // two queued positions fire with the same prefix
             uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
-            PatchRows<kBSets> pr{{0, 0}, false};
-            if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
+            PatchRows<kBSets> pr{};
+            if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
             const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.interior);
             // FP32 partial sums of at most one batch, folded into the FP64
             // prefix before an event fires and at the end of the batch
@@ -370,68 +388,53 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 pf[u] = 0.f;
               }
             };
-            while (todo) {
-              // up to kNP visible patches; a missing one repeats the first
-              int qs[kNP];
-              bool all_interior = true;
-#pragma unroll
-              for (int j = 0; j < kNP; ++j) {
-                qs[j] = todo ? __builtin_ctzll(todo) : -1;
-                todo &= todo - 1;
-                if (qs[j] >= 0) all_interior = all_interior && ((interior >> qs[j]) & 1);
-              }
-              float val[kNP][kBEPL];
+            // one patch at a time; with the row-set path the slice rows of
+            // the next patch are read while the current one is evaluated
+            // (two row buffers, used in turn)
+            const bool rows_path = ATM == PPM_ATM_NSX && we.fast;
+            auto patch_step = [&](const float (&p_cur)[1][kBSets][4], float (&p_next)[1][kBSets][4]) {
+              const int q[1] = {__builtin_ctzll(todo)};
+              todo &= todo - 1;
+              float val[1][kBEPL];
               if (ATM == PPM_ATM_NSX) {
-                float tx[kNP], w[kNP][4];
-                int im[kNP], i0[kNP][kBSets];
-#pragma unroll
-                for (int j = 0; j < kNP; ++j) {
-                  const int q = qs[j] >= 0 ? qs[j] : qs[0];
-                  tx[j] = readlane_f(pv.tx, q);
-                  im[j] = readlane_i(pv.im, q);
-#pragma unroll
-                  for (int t = 0; t < kBSets; ++t) i0[j][t] = readlane_i(pr.i0[t], q);
-#pragma unroll
-                  for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
-                }
-                if (we.fast) {
-                  if (all_interior)
-                    nsx_eval_rows<kBEPL, kNP, false, kBSets>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                if (rows_path) {
+                  if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next[0]);
+                  if ((interior >> q[0]) & 1)
+                    nsx_eval_loaded<kBEPL, 1, false, kBSets>(le, pv, pr, q, ax, p_cur, lane, val);
                   else
-                    nsx_eval_rows<kBEPL, kNP, true, kBSets>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                    nsx_eval_loaded<kBEPL, 1, true, kBSets>(le, pv, pr, q, ax, p_cur, lane, val);
                 } else {
+                  float w[4];
 #pragma unroll
-                  for (int j = 0; j < kNP; ++j) {
-                    if (qs[j] < 0) break;
-                    nsx_eval_edge<kBEPL>(le, asc, tx[j], im[j], w[j], ax, s_S, s_C, lane, val[j]);
-                  }
+                  for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[0]);
+                  nsx_eval_edge<kBEPL>(le, asc, readlane_f(pv.tx, q[0]), readlane_i(pv.im, q[0]), w, ax, s_S, s_C,
+                                       lane, val[0]);
                 }
               } else {
+                const float c = readlane_f(pv.w[0], q[0]), z = readlane_f(pv.zf, q[0]);
 #pragma unroll
-                for (int j = 0; j < kNP; ++j) {
-                  const int q = qs[j] >= 0 ? qs[j] : qs[0];
-                  const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
-#pragma unroll
-                  for (int u = 0; u < kBEPL; ++u) {
-                    const float Ee = Ef[u] * z;
-                    val[j][u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
-                  }
+                for (int u = 0; u < kBEPL; ++u) {
+                  const float Ee = Ef[u] * z;
+                  val[0][u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
                 }
               }
               // in position order: window ends at or before pos use the
               // prefix before F(pos)
-#pragma unroll
-              for (int j = 0; j < kNP; ++j) {
-                if (qs[j] < 0) break;
-                const int pos = readlane_i(k, qs[j]);
-                if (ev_pos <= pos) {
-                  fold();
-                  while (ev_pos <= pos) fire();
-                }
-#pragma unroll
-                for (int u = 0; u < kBEPL; ++u) pf[u] += val[j][u];
-                n_done += n_valid;
+              const int pos = readlane_i(k, q[0]);
+              if (ev_pos <= pos) {
+                fold();
+                while (ev_pos <= pos) fire();
               }
+#pragma unroll
+              for (int u = 0; u < kBEPL; ++u) pf[u] += val[0][u];
+              n_done += n_valid;
+            };
+            float p_a[1][kBSets][4], p_b[1][kBSets][4];
+            if (rows_path && todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a[0]);
+            while (todo) {
+              patch_step(p_a, p_b);
+              if (!todo) break;
+              patch_step(p_b, p_a);
             }
             fold();
             // drop the batch from the queue

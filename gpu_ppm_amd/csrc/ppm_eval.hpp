@@ -60,6 +60,13 @@ struct TxMap {
   float scale;  // log10(2) / h
 };
 
+// flux/I of the patch at rotational phase pp, with the fine-table bracket
+// (i, a) it was interpolated in.  Zero exactly where the patch is skipped.
+__device__ inline double patch_flux_factor(double pp, const RingTab& R, int& i, double& a) {
+  i = vec_hunt(R.ph, R.nf1, pp, a);
+  return (1 - a) * (double)R.fI[i] + a * (double)R.fI[i + 1];
+}
+
 template <int ATM>
 __device__ inline void patch_setup(double pp, double weight, const RingTab& R, const double* s_mu,
                                    const double* mu_den, int nm, double mu_min, TxMap txm, PatchVals& o) {
@@ -68,8 +75,8 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
   o.zf = 0.f;
   o.im = -1;
   double a;
-  const int i = vec_hunt(R.ph, R.nf1, pp, a);
-  const double fI = (1 - a) * (double)R.fI[i] + a * (double)R.fI[i + 1];
+  int i;
+  const double fI = patch_flux_factor(pp, R, i, a);
   if (fI == 0) return;
   const double zz = (1 - a) * (double)R.zf[i] + a * (double)R.zf[i + 1];
   const double c = fI * weight;
@@ -106,6 +113,7 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
 struct EAxis {
   float t_lo, t_hi;  // local t clamped to the axis ends
   int st_lo, st_hi;  // legal local stencil starts (also the staged range)
+  int i0_max;        // largest row-set start whose 64 rows lie in the staged allocation
 };
 
 // Intensity at local axis position t (cubic Lagrange over 4 rows x 4 mu
@@ -230,23 +238,36 @@ __device__ inline WaveEnergies<SETS> wave_energies(const float (&le)[EPL], int l
 // evaluated there: without clamps when no energy reaches a clamp of the axis
 // or of the stencil start (checked on the first and last energy of each set,
 // the evaluation being monotone in the energy), with them otherwise.
-// i0[s] - 1 is the stencil start of the first energy of set s.
+// Set s holds slice rows i0[s] - 1 + lane: i0[s] is at most the stencil
+// start + 1 of the set's first energy, and at most EAxis::i0_max so that
+// every lane's row is in the staged allocation.  txs = tx - i0 puts a set's
+// energies on its lane axis (floor(le + txs) is the lane holding the row),
+// rb = (i0 - 1) * kSliceStride + im is the LDS offset of lane 0's row at the
+// patch's first mu column.
 template <int SETS>
 struct PatchRows {
   int i0[SETS];
+  float txs[SETS];
+  int rb[SETS];
   bool interior;  // no clamps
 };
 
 template <int SETS>
-__device__ inline PatchRows<SETS> patch_rows(float tx, const WaveEnergies<SETS>& we, const EAxis& ax) {
+__device__ inline PatchRows<SETS> patch_rows(float tx, int im, const WaveEnergies<SETS>& we, const EAxis& ax) {
   PatchRows<SETS> r;
   r.interior = we.fast;
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
-    const float tf = we.le_first[s] + tx, tl = we.le_last[s] + tx;
-    r.interior = r.interior && tf >= 1.f && tl <= ax.t_hi && tl < (float)(ax.st_hi + 2);
-    const float tc = __builtin_amdgcn_fmed3f(tf, ax.t_lo, ax.t_hi);
-    r.i0[s] = med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1);
+    const float tc = __builtin_amdgcn_fmed3f(we.le_first[s] + tx, ax.t_lo, ax.t_hi);
+    r.i0[s] = min(med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1), ax.i0_max);
+    r.txs[s] = tx - (float)r.i0[s];
+    r.rb[s] = (r.i0[s] - 1) * kSliceStride + im;
+    // the unclamped path evaluates at le + txs (rounded once): its end
+    // points, taken the same way, bound every energy's lane and stencil
+    // (the rounding of le + tx may differ near an integer)
+    const float uf = we.le_first[s] + r.txs[s], ul = we.le_last[s] + r.txs[s];
+    const float i0f = (float)r.i0[s];
+    r.interior = r.interior && uf >= 0.f && ul < (float)(ax.st_hi + 2) - i0f && ul <= ax.t_hi - i0f;
   }
   return r;
 }
@@ -265,76 +286,108 @@ __device__ inline int floor_to_int(float t) {
 
 // Contracted evaluation of NP patches at once (independent chains, for
 // latency hiding), every patch of a wavefront with WaveEnergies::fast and,
-// with CLAMP = false, PatchRows::interior.  For energy t = le + tx (clamped
-// to the axis ends) the stencil start is st = floor(t) - 1 (clamped to the
-// legal starts) and x = t - (st + 1), in [0, 1) away from the clamps.  For
-// each row set, lane l contracts slice row i0 - 1 + l over the patch's mu
-// stencil, takes the next three rows from its neighbours and forms the
-// power-basis coefficients of the cubic through the four rows at the nodes
-// x = -1, 0, 1, 2:
+// with CLAMP = false, PatchRows::interior; patch j is the one set up in lane
+// q[j] (pv, pr).  For energy t = le + tx (clamped to the axis ends) the
+// stencil start is st = floor(t) - 1 (clamped to the legal starts) and
+// x = t - (st + 1), in [0, 1) away from the clamps.  For each row set, lane
+// l contracts slice row i0 - 1 + l over the patch's mu stencil, takes the
+// next three rows from its neighbours and forms the power-basis coefficients
+// of the cubic through the four rows at the nodes x = -1, 0, 1, 2:
 //   I(x) = a0 + x (a1 + x (a2 + x a3))
 // (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256); each
 // energy then fetches the coefficients of its row from the lane that holds
-// them.  `n_rows` is the number of staged slice rows.
+// them.  The differences of lane l + 1 are the next differences of lane l,
+// so each order takes one shifted subtraction.  Away from the clamps an
+// energy's lane and x come from t - i0 = le + txs directly.
+//
+// nsx_load_rows issues the slice-row reads of patch q (lane l: row i0 - 1 + l
+// of each set at the patch's four mu columns); a caller may issue them for
+// the next patch before evaluating the current one.
+template <int SETS>
+__device__ inline void nsx_load_rows(const PatchRows<SETS>& pr, int q, const float* s_S, int lane,
+                                     float (&p)[SETS][4]) {
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
+    const float* r = s_S + readlane_i(pr.rb[s], q) + lane * kSliceStride;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) p[s][m] = r[m];
+  }
+}
+
 template <int EPL, int NP, bool CLAMP, int SETS>
-__device__ inline void nsx_eval_rows(const float (&le)[EPL], const float (&tx)[NP], const int (&im)[NP],
-                                     const int (&i0)[NP][SETS], const float (&w)[NP][4], const EAxis& ax,
-                                     int n_rows, const float* s_S, int lane, float (&out)[NP][EPL]) {
+__device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
+                                       const int (&q)[NP], const EAxis& ax, const float (&p)[NP][SETS][4], int lane,
+                                       float (&out)[NP][EPL]) {
   constexpr int G = EPL / SETS;
-  // all LDS reads of a stage are issued before their results are used
-  float p[NP][SETS][4];
-#pragma unroll
-  for (int j = 0; j < NP; ++j)
-#pragma unroll
-    for (int s = 0; s < SETS; ++s) {
-      const int row = min(i0[j][s] - 1 + lane, n_rows - 1);
-      const float* r = s_S + __mul24(row, kSliceStride) + im[j];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) p[j][s][m] = r[m];
-    }
   float a0[NP][SETS], a1[NP][SETS], a2[NP][SETS], a3[NP][SETS];
 #pragma unroll
-  for (int j = 0; j < NP; ++j)
+  for (int j = 0; j < NP; ++j) {
+    float w[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[j]);
 #pragma unroll
     for (int s = 0; s < SETS; ++s) {
-      const float U = w[j][0] * p[j][s][0] + w[j][1] * p[j][s][1] + w[j][2] * p[j][s][2] + w[j][3] * p[j][s][3];
+      const float U = w[0] * p[j][s][0] + w[1] * p[j][s][1] + w[2] * p[j][s][2] + w[3] * p[j][s][3];
       const float U1 = lane_next(U);
-      const float U2 = lane_next(U1);
-      const float U3 = lane_next(U2);
-      const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
-      const float D2 = d1b - d1, D3 = (d1c - d1b) - D2;
+      const float d1 = U1 - U;                // U(0) - U(-1)
+      const float D2 = lane_next(d1) - d1;    // second difference at x = 0
+      const float D3 = lane_next(D2) - D2;    // third difference
       a2[j][s] = 0.5f * D2;
       a3[j][s] = (1.f / 6.f) * D3;
       a1[j][s] = d1 + a2[j][s] - a3[j][s];
       a0[j][s] = U1;
     }
+  }
   float x[NP][EPL], c[NP][EPL][4];
 #pragma unroll
-  for (int j = 0; j < NP; ++j)
+  for (int j = 0; j < NP; ++j) {
+    float tx = 0.f, txs[SETS];
+    int i0[SETS];
+#pragma unroll
+    for (int s = 0; s < SETS; ++s) {
+      if (CLAMP) {
+        i0[s] = readlane_i(pr.i0[s], q[j]);
+      } else {
+        txs[s] = readlane_f(pr.txs[s], q[j]);
+      }
+    }
+    if (CLAMP) tx = readlane_f(pv.tx, q[j]);
 #pragma unroll
     for (int u = 0; u < EPL; ++u) {
       const int s = u / G;
-      int f1;
+      int src;
       if (CLAMP) {
-        const float t = __builtin_amdgcn_fmed3f(le[u] + tx[j], ax.t_lo, ax.t_hi);
-        f1 = med3i(floor_to_int(t), ax.st_lo + 1, ax.st_hi + 1);
+        const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
+        const int f1 = med3i(floor_to_int(t), ax.st_lo + 1, ax.st_hi + 1);
         x[j][u] = t - (float)f1;
+        src = (f1 - i0[s]) << 2;
       } else {
-        const float t = le[u] + tx[j];
-        f1 = floor_to_int(t);
+        const float t = le[u] + txs[s];
+        src = floor_to_int(t) << 2;
         x[j][u] = __builtin_amdgcn_fractf(t);
       }
-      const int src = (f1 - i0[j][s]) << 2;
       c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j][s])));
       c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j][s])));
       c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j][s])));
       c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j][s])));
     }
+  }
 #pragma unroll
   for (int j = 0; j < NP; ++j)
 #pragma unroll
     for (int u = 0; u < EPL; ++u)
       out[j][u] = fmaf(x[j][u], fmaf(x[j][u], fmaf(x[j][u], c[j][u][3], c[j][u][2]), c[j][u][1]), c[j][u][0]);
+}
+
+template <int EPL, int NP, bool CLAMP, int SETS>
+__device__ inline void nsx_eval_rows(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
+                                     const int (&q)[NP], const EAxis& ax, const float* s_S, int lane,
+                                     float (&out)[NP][EPL]) {
+  // all LDS reads of a stage are issued before their results are used
+  float p[NP][SETS][4];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) nsx_load_rows(pr, q[j], s_S, lane, p[j]);
+  nsx_eval_loaded<EPL, NP, CLAMP, SETS>(le, pv, pr, q, ax, p, lane, out);
 }
 
 // One patch whose wavefront energies span too many rows for nsx_eval_rows:

@@ -478,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
   const double inv_h = 1.0 / h;
   const double l_ref = log10(P.d_energies[e_first]);
   int r_lo = 0, n_rows = 0;
-  EAxis ax{0.f, 0.f, 0, 0};
+  EAxis ax{0.f, 0.f, 0, 0, 1};
   if (ATM == PPM_ATM_NSX) {
     const double lkT = log10(S.kT_nsx);
     const double t_min = (l_ref + log10(Dp->zf_min) - lkT - N.logE_min) * inv_h;
@@ -494,6 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
     ax.t_hi = (float)(N.nE - 1 - r_lo);
     ax.st_lo = 0;
     ax.st_hi = min(N.nE - 4 - r_lo, n_rows - 4);
+    ax.i0_max = rows_alloc - 63;  // rows_alloc >= 64
     for (int k = tid; k < N.nm; k += kBlock) s_mu[k] = N.mu[k];
     stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBlock);
   }
@@ -559,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
 #pragma unroll
         for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
         uint64_t todo = __ballot(pv.im >= 0);
-        PatchRows<1> pr{{0}, false};
-        if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, we, ax);
+        PatchRows<1> pr{};
+        if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
         const uint64_t interior = __ballot(pv.im >= 0 && pr.interior);
         while (todo) {
           int qs[kNP];  // a missing patch repeats the first
@@ -572,28 +573,24 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
             if (qs[j] >= 0) all_interior = all_interior && ((interior >> qs[j]) & 1);
           }
           if (ATM == PPM_ATM_NSX) {
-            float tx[kNP], w[kNP][4];
-            int im[kNP], i0[kNP][1];
+            int q[kNP];
 #pragma unroll
-            for (int j = 0; j < kNP; ++j) {
-              const int q = qs[j] >= 0 ? qs[j] : qs[0];
-              tx[j] = readlane_f(pv.tx, q);
-              im[j] = readlane_i(pv.im, q);
-              i0[j][0] = readlane_i(pr.i0[0], q);
-#pragma unroll
-              for (int m = 0; m < 4; ++m) w[j][m] = readlane_f(pv.w[m], q);
-            }
+            for (int j = 0; j < kNP; ++j) q[j] = qs[j] >= 0 ? qs[j] : qs[0];
             float val[kNP][kEPL];
             if (we.fast) {
               if (all_interior)
-                nsx_eval_rows<kEPL, kNP, false, 1>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, false, 1>(le, pv, pr, q, ax, s_S, lane, val);
               else
-                nsx_eval_rows<kEPL, kNP, true, 1>(le, tx, im, i0, w, ax, n_rows, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, true, 1>(le, pv, pr, q, ax, s_S, lane, val);
             } else {
 #pragma unroll
               for (int j = 0; j < kNP; ++j) {
                 if (qs[j] < 0) break;
-                nsx_eval_edge<kEPL>(le, asc, tx[j], im[j], w[j], ax, s_S, s_C, lane, val[j]);
+                float w[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[j]);
+                nsx_eval_edge<kEPL>(le, asc, readlane_f(pv.tx, q[j]), readlane_i(pv.im, q[j]), w, ax, s_S, s_C, lane,
+                                    val[j]);
               }
             }
 #pragma unroll
