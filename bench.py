@@ -98,7 +98,7 @@ def timed(ctx, stars, g, d_flux, d_cnt, steps, warmup):
     return time.perf_counter() - t0, stats
 
 
-def bench_batched(root, steps, warmup, n_stars, plain_stars):
+def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
     from gpu_ppm_amd import abi, hip
     from gpu_ppm_amd import scenarios as S
     from gpu_ppm_amd.lib import Context
@@ -108,6 +108,13 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars):
     d_flux = hip.DeviceArray((n_stars, N_PHASE, N_E))
     d_cnt = hip.DeviceArray((n_stars, n_ch, N_PHASE))
     gb = grid(abi.METHOD_BATCHED)
+    shape = None
+    if tune:  # the launch shape the autotuner keeps for this grid shape (outside the timed region)
+        log("bench: autotuning the batched launch shape on the %d stars" % n_stars)
+        tr = ctx.tune(stars, gb, repeats=1)
+        shape = {"items_per_slot": tr.best.items_per_slot, "wg_per_cu": tr.best.wg_per_cu,
+                 "tuned_integral_ms": tr.best_ms, "default_integral_ms": tr.default_ms, "shapes_tried": tr.n_tried}
+        log("bench: tuned shape %s" % shape)
     log("bench: batched %d stars, warmup %d, steps %d" % (n_stars, warmup, steps))
     dt, st = timed(ctx, stars, gb, d_flux, d_cnt, steps, warmup)
     log("bench: batched %.3f s per step" % (dt / steps))
@@ -115,6 +122,8 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars):
     # per-kernel time of the batched integral: one direct (non-graph) launch
     os.environ["PPM_NO_GRAPH"] = "1"
     ctx = Context(root, lensing="wide", responses=("std",))
+    if shape:
+        ctx.set_launch_shape(gb, n_stars, abi.ATM_NSX, shape["items_per_slot"], shape["wg_per_cu"])
     _, sk = timed(ctx, stars, gb, d_flux, d_cnt, 1, 0)
     # plain path on the same stars, one profile per launch sequence
     gp = grid(abi.METHOD_PLAIN)
@@ -140,7 +149,8 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars):
                 n_eval_ref=k.n_eval_ref, n_active=k.n_active,
                 plain_profiles_per_s=len(sub) / t_plain, plain_stars=len(sub),
                 plain_evals_per_s=ref_evals / (plain_int_ms * 1e-3),
-                graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms)
+                graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms,
+                launch_shape=shape)
 
 
 def bench_serving(root, duration_s, burst, seed=7):
@@ -262,8 +272,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched", choices=["batched", "plain"])
     ap.add_argument("--stars", type=int, default=256, help="stars per batched step")
-    ap.add_argument("--plain-stars", type=int, default=64, help="stars timed on the plain path")
+    ap.add_argument("--plain-stars", type=int, default=256,
+                    help="stars timed on the plain path (default: the whole batched workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the default batched launch shape")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="duration of the open-loop serving leg (0 skips it)")
     ap.add_argument("--serve-burst", type=int, default=192, help="requests in the capacity burst")
@@ -280,7 +292,7 @@ def main():
     root = data_root()
     log("bench: data ready; workload", args.workload)
     if args.workload == "batched":
-        r = bench_batched(root, args.steps, args.warmup, args.stars, args.plain_stars)
+        r = bench_batched(root, args.steps, args.warmup, args.stars, args.plain_stars, tune=not args.no_tune)
         cfg = {"workload": "batched: %d prior-drawn stars per launch, N_side 1024, 64 bins x 2 GL, 360 E, NSX, "
                            "fold 300x360" % args.stars, "n_stars": args.stars}
         unit_note = "%d distinct evaluations (phase-shift dedupe) x %d FLOP" % (r["n_eval"], FLOP_PER_EVAL)
@@ -319,7 +331,8 @@ def main():
         out["batched_vs_plain"] = r["value"] / r["plain_profiles_per_s"]
         out["batched"] = {"graph_ms_per_step": r["graph_ms"], "integral_ms": r["kernel_ms"],
                           "setup_reduce_fold_ms": r["setup_reduce_fold_ms"], "n_eval_ref": r["n_eval_ref"],
-                          "dedupe_factor": r["n_eval_ref"] / max(r["n_eval"], 1)}
+                          "dedupe_factor": r["n_eval_ref"] / max(r["n_eval"], 1),
+                          "launch_shape": r["launch_shape"]}
     if args.workload == "batched" and args.serve_seconds > 0:
         out["serving"] = bench_serving(root, args.serve_seconds, args.serve_burst)
     if not args.no_cpu:

@@ -92,6 +92,15 @@ class Stats(ctypes.Structure):
     ]
 
 
+class LaunchShape(ctypes.Structure):
+    _fields_ = [("items_per_slot", c_int), ("wg_per_cu", c_int)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best", LaunchShape), ("best_ms", c_double), ("default_ms", c_double), ("n_tried", c_int),
+                ("reserved", c_int)]
+
+
 NSX_H_V200804 = dict(n_logT=35, n_logg=14, n_logEkT=166, n_mu=67, logT_min=5.10, logT_max=6.80,
                      logg_min=13.7, logg_max=15.0, logEkT_min=-1.30, logEkT_max=2.00)
 

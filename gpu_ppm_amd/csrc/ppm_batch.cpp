@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 
 #include "ppm_batch_dev.hpp"
@@ -81,12 +82,30 @@ struct GraphCache {
   std::vector<unsigned char> key, last_direct;
 };
 
+// Launch shape of the batched kernel: work-queue items per resident
+// workgroup (how finely each spot's rings are split; more items balance the
+// persistent workgroups better, fewer carry less per-item set-up) and
+// resident workgroups per CU.
+constexpr int kDefaultItemsPerSlot = 12;
+
 struct BatchState {
   ClassCache classes;
   GraphCache graph;
   DevBuf d_csets, d_chdr, d_cpts, d_order, d_star_item0, d_acc, d_seg;
   int n_cu = 0;
+  // shapes found by ppm_tune or set by the caller, per shape_key
+  std::map<std::vector<int>, ppm_launch_shape> shapes;
+  const ppm_launch_shape* forced = nullptr;  // set while ppm_tune times a candidate
+  int occ_wg_max = 0;                        // occupancy limit of the last launch
 };
+
+// The grid fields that shape the batched work, the atmosphere and the bit
+// length of the star count.
+static std::vector<int> shape_key(const ppm_grid& g, int n_e, int atm, int n) {
+  int bits = 0;
+  while (bits < 31 && (1 << bits) <= n) ++bits;
+  return {g.n_side, g.n_phase, g.sampling, g.sampling == PPM_SAMPLE_GL ? g.n_gl : 0, g.n_fine, n_e, atm, bits};
+}
 
 static BatchState* state(ppm_ctx* c) {
   if (!c->batch_state) c->batch_state = new BatchState();
@@ -231,6 +250,15 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   if ((e = batched_prepare(atm, lds))) return hip_fail(c, e, "kernel attributes");
   int wg_per_cu = 0;
   if (batched_occupancy(atm, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
+  BS->occ_wg_max = wg_per_cu;
+  ppm_launch_shape shape{0, 0};
+  if (BS->forced) {
+    shape = *BS->forced;
+  } else {
+    auto it = BS->shapes.find(shape_key(g, n_e, atm, n));
+    if (it != BS->shapes.end()) shape = it->second;
+  }
+  if (shape.wg_per_cu > 0) wg_per_cu = std::min(wg_per_cu, shape.wg_per_cu);
   const int slots = wg_per_cu * BS->n_cu;
 
   // ---- items: each spot's rings split into chunks of similar cost
@@ -238,7 +266,8 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   for (double v : cost) total_cost += v;
   const size_t acc_per_item = (size_t)g.n_phase * n_e * batch_acc_copies();
   const size_t cap_items = std::max<size_t>(1, ((size_t)3072 << 20) / (acc_per_item * sizeof(double)));
-  const double target = std::min<double>((double)cap_items / n_echunk, 12.0 * slots / n_echunk);
+  const double per_slot = shape.items_per_slot > 0 ? shape.items_per_slot : kDefaultItemsPerSlot;
+  const double target = std::min<double>((double)cap_items / n_echunk, per_slot * slots / n_echunk);
   std::vector<BatchItem> items;
   std::vector<double> item_cost;
   std::vector<int> star_item0(n + 1, 0);
@@ -445,4 +474,153 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   return PPM_OK;
 }
 
+// ---- autotuner of the batched launch shape
+
+namespace {
+
+ppm_status shape_args(ppm_ctx* c, const ppm_grid* grid, int n_stars, int atmosphere, std::vector<int>& key) {
+  if (!grid || n_stars < 1) return fail(c, PPM_ERR_ARG, "null grid or no stars");
+  if (atmosphere != PPM_ATM_BLACKBODY && atmosphere != PPM_ATM_NSX) return fail(c, PPM_ERR_ARG, "unknown atmosphere");
+  std::vector<double> energies;
+  ppm_status s = resolve_energies(c, *grid, energies);
+  if (s) return s;
+  key = shape_key(*grid, (int)energies.size(), atmosphere, n_stars);
+  return PPM_OK;
+}
+
+}  // namespace
+
 }  // namespace ppm
+
+using namespace ppm;
+
+extern "C" {
+
+ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, int repeats, ppm_tune_result* out) {
+  if (!c || !stars || !grid || n < 1 || !out) return fail(c, PPM_ERR_ARG, "null argument");
+  if (repeats < 0 || repeats > 16) return fail(c, PPM_ERR_ARG, "repeats must be in 0..16");
+  if (repeats == 0) repeats = 2;
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  hipSetDevice(c->device);
+  std::vector<int> key;
+  ppm_status s = shape_args(c, grid, n, stars[0].atmosphere, key);
+  if (s) return s;
+  // the grid's energies, given explicitly: the fold (which does not depend
+  // on the shape) is not run
+  std::vector<double> energies;
+  if ((s = resolve_energies(c, *grid, energies))) return s;
+  ppm_grid g = *grid;
+  g.method = PPM_METHOD_BATCHED;
+  g.energies_keV = energies.data();
+  g.n_energy = (int)energies.size();
+  g.response = -1;
+  const size_t nf = (size_t)n * g.n_phase * key[5];
+  DevBuf d_flux;
+  int e = d_flux.ensure(sizeof(double) * nf);
+  if (e) {
+    d_flux.release();
+    return hip_fail(c, e, "device allocation");
+  }
+  BatchState* BS = state(c);
+  ppm_launch_shape cur{0, 0};
+  // direct launches (the integral's own events) and the candidate shape,
+  // restored on every exit
+  struct Restore {
+    ppm_ctx* c;
+    BatchState* BS;
+    DevBuf* buf;
+    bool no_graph;
+    ~Restore() {
+      c->no_graph = no_graph;
+      BS->forced = nullptr;
+      hipStreamSynchronize(c->stream);
+      buf->release();
+    }
+  } restore{c, BS, &d_flux, c->no_graph};
+  c->no_graph = true;
+  BS->forced = &cur;
+  int n_tried = 0;
+  auto time_shape = [&](ppm_launch_shape sh, double& ms) -> ppm_status {
+    cur = sh;
+    ms = 1e300;
+    for (int r = 0; r < repeats; ++r) {
+      ppm_stats st{};
+      const ppm_status q = eval_any(c, stars, n, &g, (double*)d_flux.p, nullptr, &st);
+      if (q) return q;
+      ms = std::min(ms, st.integral_ms);
+    }
+    ++n_tried;
+    return PPM_OK;
+  };
+  // untimed warm-up (allocations, first launches); it also gives the
+  // occupancy limit
+  {
+    cur = ppm_launch_shape{0, 0};
+    ppm_stats st{};
+    if ((s = eval_any(c, stars, n, &g, (double*)d_flux.p, nullptr, &st))) return s;
+  }
+  const int wg_max = std::max(1, BS->occ_wg_max);
+  ppm_launch_shape best{kDefaultItemsPerSlot, wg_max};
+  double t_best, t_def;
+  if ((s = time_shape(best, t_def))) return s;
+  t_best = t_def;
+  auto consider = [&](ppm_launch_shape sh) -> ppm_status {
+    double t;
+    const ppm_status q = time_shape(sh, t);
+    if (q) return q;
+    if (t < t_best) {
+      t_best = t;
+      best = sh;
+    }
+    return PPM_OK;
+  };
+  // items per workgroup: halve and double, then one more step the better way
+  for (int f : {kDefaultItemsPerSlot / 2, kDefaultItemsPerSlot * 2})
+    if ((s = consider(ppm_launch_shape{f, wg_max}))) return s;
+  if (best.items_per_slot != kDefaultItemsPerSlot) {
+    const int f = best.items_per_slot < kDefaultItemsPerSlot ? best.items_per_slot / 2 : best.items_per_slot * 2;
+    if ((s = consider(ppm_launch_shape{std::max(f, 1), wg_max}))) return s;
+  }
+  // one resident workgroup fewer per CU
+  if (wg_max > 1 && (s = consider(ppm_launch_shape{best.items_per_slot, wg_max - 1}))) return s;
+  BS->shapes[key] = best;
+  out->best = best;
+  out->best_ms = t_best;
+  out->default_ms = t_def;
+  out->n_tried = n_tried;
+  out->reserved = 0;
+  return PPM_OK;
+}
+
+ppm_status ppm_set_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, int atmosphere,
+                                const ppm_launch_shape* shape) {
+  if (!c) return PPM_ERR_ARG;
+  if (shape && (shape->items_per_slot < 0 || shape->items_per_slot > 4096 || shape->wg_per_cu < 0 ||
+                shape->wg_per_cu > 64))
+    return fail(c, PPM_ERR_ARG, "launch shape out of range (items_per_slot 0..4096, wg_per_cu 0..64)");
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  std::vector<int> key;
+  ppm_status s = shape_args(c, grid, n_stars, atmosphere, key);
+  if (s) return s;
+  BatchState* BS = state(c);
+  if (shape && (shape->items_per_slot || shape->wg_per_cu))
+    BS->shapes[key] = *shape;
+  else
+    BS->shapes.erase(key);
+  return PPM_OK;
+}
+
+ppm_status ppm_get_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, int atmosphere,
+                                ppm_launch_shape* shape) {
+  if (!c || !shape) return PPM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  std::vector<int> key;
+  ppm_status s = shape_args(c, grid, n_stars, atmosphere, key);
+  if (s) return s;
+  BatchState* BS = state(c);
+  auto it = BS->shapes.find(key);
+  *shape = it != BS->shapes.end() ? it->second : ppm_launch_shape{0, 0};
+  return PPM_OK;
+}
+
+}  // extern "C"

@@ -68,6 +68,11 @@ def load():
     L.ppm_cancel.argtypes = [c_void_p, c_uint64]
     L.ppm_drain.restype = c_int
     L.ppm_drain.argtypes = [c_void_p]
+    L.ppm_tune.restype = c_int
+    L.ppm_tune.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), c_int, POINTER(abi.TuneResult)]
+    for f in (L.ppm_set_launch_shape, L.ppm_get_launch_shape):
+        f.restype = c_int
+        f.argtypes = [c_void_p, POINTER(abi.Grid), c_int, c_int, POINTER(abi.LaunchShape)]
     _lib = L
     return L
 
@@ -184,6 +189,25 @@ class Context:
         sc = np.ascontiguousarray(scale, dtype=np.float64)
         self._check(load().ppm_fold_device(self._h, response, n_batch, n_phase, c_void_p(d_flux),
                                            _dptr(sc), c_void_p(d_counts)))
+
+    # -- launch shape of the batched path
+    def tune(self, stars, grid, repeats=0):
+        """Autotune the batched launch shape for this grid shape on these
+        stars; returns abi.TuneResult (the shape is kept in the context)."""
+        arr = _star_array(stars)
+        r = abi.TuneResult()
+        self._check(load().ppm_tune(self._h, arr, len(arr), byref(grid), repeats, byref(r)))
+        return r
+
+    def set_launch_shape(self, grid, n_stars, atmosphere, items_per_slot=0, wg_per_cu=0):
+        sh = abi.LaunchShape(items_per_slot, wg_per_cu)
+        self._check(load().ppm_set_launch_shape(self._h, byref(grid), n_stars, atmosphere, byref(sh)))
+
+    def launch_shape(self, grid, n_stars, atmosphere):
+        """(items_per_slot, wg_per_cu) stored for this grid shape; (0, 0): none."""
+        sh = abi.LaunchShape()
+        self._check(load().ppm_get_launch_shape(self._h, byref(grid), n_stars, atmosphere, byref(sh)))
+        return sh.items_per_slot, sh.wg_per_cu
 
     # -- serving
     def submit(self, star, grid):

@@ -203,6 +203,43 @@ ppm_status ppm_cancel(ppm_ctx* ctx, uint64_t ticket);
 /* Block until every admitted request is finished or cancelled. */
 ppm_status ppm_drain(ppm_ctx* ctx);
 
+/* Launch shape of the batched path.  A 0 field means the default: 12
+ * work-queue items per resident workgroup (each spot's rings are cut into
+ * chunks of similar cost so that the queue holds about that many items per
+ * workgroup), and the occupancy limit of workgroups per CU. */
+typedef struct ppm_launch_shape {
+  int items_per_slot; /* work-queue items per resident workgroup (0..4096) */
+  int wg_per_cu;      /* resident workgroups per CU (0..64, capped by occupancy) */
+} ppm_launch_shape;
+
+typedef struct ppm_tune_result {
+  ppm_launch_shape best; /* the shape kept for this grid shape            */
+  double best_ms;        /* its integral time (min over repeats, hipEvent) */
+  double default_ms;     /* the default shape's integral time             */
+  int n_tried;           /* shapes timed                                  */
+  int reserved;
+} ppm_tune_result;
+
+/* Autotuner of the batched launch.  No counterpart in the reference, whose
+ * launch geometry is fixed (one block per ring, gpu/devarshi24st_gpu.cu:463,
+ * gpu/os1_gpu.cu:316).  Times the batched integral of `stars` on `grid` over
+ * a short coordinate search of the launch shape (items per workgroup halved
+ * and doubled, one more step the better way, then one workgroup per CU
+ * fewer), `repeats` runs each (0 -> 2), and keeps the fastest for this grid
+ * shape (grid fields, energy count, atmosphere of stars[0], bit length of
+ * n): later batched evaluations and served launches of that shape use it.
+ * Blocking; the fold is not run. */
+ppm_status ppm_tune(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid, int repeats,
+                    ppm_tune_result* out);
+
+/* Store a shape for a grid shape (for example one saved from an earlier
+ * ppm_tune), or clear it (shape NULL or all zero); read the stored shape
+ * (all zero: none stored, the default applies). */
+ppm_status ppm_set_launch_shape(ppm_ctx* ctx, const ppm_grid* grid, int n_stars, int atmosphere,
+                                const ppm_launch_shape* shape);
+ppm_status ppm_get_launch_shape(ppm_ctx* ctx, const ppm_grid* grid, int n_stars, int atmosphere,
+                                ppm_launch_shape* shape);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,15 +33,18 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include "ppm.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ppm_spot), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ppm_spot), '
                    'sizeof(ppm_star), sizeof(ppm_grid), sizeof(ppm_nsx_config), sizeof(ppm_open_opts), '
-                   'sizeof(ppm_stats), offsetof(ppm_star, spot), offsetof(ppm_grid, energies_keV));return 0;}\n')
+                   'sizeof(ppm_stats), offsetof(ppm_star, spot), offsetof(ppm_grid, energies_keV), '
+                   'sizeof(ppm_launch_shape), sizeof(ppm_tune_result), offsetof(ppm_tune_result, n_tried));'
+                   'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     want = [ctypes.sizeof(abi.Spot), ctypes.sizeof(abi.Star), ctypes.sizeof(abi.Grid),
             ctypes.sizeof(abi.NsxConfig), ctypes.sizeof(abi.OpenOpts), ctypes.sizeof(abi.Stats),
-            abi.Star.spot.offset, abi.Grid.energies_keV.offset]
+            abi.Star.spot.offset, abi.Grid.energies_keV.offset, ctypes.sizeof(abi.LaunchShape),
+            ctypes.sizeof(abi.TuneResult), abi.TuneResult.n_tried.offset]
     assert got == want
 
 
@@ -125,3 +128,16 @@ def test_context_raises_loudly(tmp_path):
     with pytest.raises(lib.PpmError) as e:
         lib.Context(root=str(tmp_path))
     assert e.value.status == abi.ERR_IO
+
+
+def test_tuner_entry_points_reject_bad_arguments():
+    """The autotuner's entry points fail with a status on a null context
+    (no device needed)."""
+    L = lib.load()
+    g = abi.make_grid(16, 8)
+    st = abi.make_star([abi.make_spot(0.5, 0.2, 0.3)])
+    r = abi.TuneResult()
+    sh = abi.LaunchShape(12, 0)
+    assert L.ppm_tune(None, ctypes.byref(st), 1, ctypes.byref(g), 0, ctypes.byref(r)) == abi.ERR_ARG
+    assert L.ppm_set_launch_shape(None, ctypes.byref(g), 1, abi.ATM_NSX, ctypes.byref(sh)) == abi.ERR_ARG
+    assert L.ppm_get_launch_shape(None, ctypes.byref(g), 1, abi.ATM_NSX, ctypes.byref(sh)) == abi.ERR_ARG
