@@ -68,10 +68,16 @@ void build_classes(int n_side, const std::vector<PhasePoint>& pp, ClassTables& T
 
 }  // namespace
 
+// Class tables of one grid's phase points, on the host and the device.
 struct ClassCache {
-  int n_side = -1, n_phase = -1, sampling = -1, n_gl = -1;
   ClassTables tab;
-  bool uploaded = false;
+  DevBuf csets, chdr, cpts;
+  uint64_t last_use = 0;
+  void release() {
+    csets.release();
+    chdr.release();
+    cpts.release();
+  }
 };
 
 // The launch sequence is captured into a graph the second time the same
@@ -82,6 +88,9 @@ struct GraphCache {
   std::vector<unsigned char> key, last_direct;
 };
 
+// Device memory of the item accumulators of one launch, at most.
+constexpr size_t kAccCapBytes = (size_t)12 << 30;
+
 // Launch shape of the batched kernel: work-queue items per resident
 // workgroup (how finely each spot's rings are split; more items balance the
 // persistent workgroups better, fewer carry less per-item set-up) and
@@ -89,9 +98,10 @@ struct GraphCache {
 constexpr int kDefaultItemsPerSlot = 12;
 
 struct BatchState {
-  ClassCache classes;
+  std::map<std::vector<int>, ClassCache> classes;  // per (n_side, n_phase, sampling, n_gl)
+  uint64_t use_clock = 0;
   GraphCache graph;
-  DevBuf d_csets, d_chdr, d_cpts, d_order, d_star_item0, d_acc, d_seg;
+  DevBuf d_grids, d_order, d_star_item0, d_acc, d_seg;
   int n_cu = 0;
   // shapes found by ppm_tune or set by the caller, per shape_key
   std::map<std::vector<int>, ppm_launch_shape> shapes;
@@ -135,194 +145,328 @@ void release_batch_state(ppm_ctx* c) {
   BatchState* S = c->batch_state;
   if (!S) return;
   if (S->graph.exec) hipGraphExecDestroy(S->graph.exec);
-  for (DevBuf* b : {&S->d_csets, &S->d_chdr, &S->d_cpts, &S->d_order, &S->d_star_item0, &S->d_acc, &S->d_seg})
-    b->release();
+  for (auto& kv : S->classes) kv.second.release();
+  for (DevBuf* b : {&S->d_grids, &S->d_order, &S->d_star_item0, &S->d_acc, &S->d_seg}) b->release();
   delete S;
   c->batch_state = nullptr;
 }
 
-ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
-                             const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
-                             double* d_flux, double* d_counts, ppm_stats* stats, int rows_override);
+static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats, int rows_override);
 
 ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats) {
-  return eval_batched_rows(c, stars, n, g, energies, pts, d_flux, d_counts, stats, 0);
+  std::vector<BatchGroup> groups(1);
+  BatchGroup& G = groups[0];
+  G.stars = stars;
+  G.n = n;
+  G.g = g;
+  G.energies = energies;
+  G.pts = pts;
+  G.d_flux = d_flux;
+  G.d_counts = d_counts;
+  return eval_groups_rows(c, groups, stats, 0);
 }
 
-ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
-                             const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
-                             double* d_flux, double* d_counts, ppm_stats* stats, int rows_override) {
+ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats) {
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  return eval_groups_rows(c, groups, stats, 0);
+}
+
+// Class tables of a grid's phase points, built and uploaded once per grid
+// (kMaxClassGrids kept; the least recently used one not used by the current
+// launch is dropped).
+constexpr size_t kMaxClassGrids = 24;
+
+static ppm_status class_tables(ppm_ctx* c, BatchState* BS, const ppm_grid& g, const std::vector<PhasePoint>& pts,
+                               uint64_t launch_clock, ClassCache*& out) {
+  const std::vector<int> key{g.n_side, g.n_phase, g.sampling, g.sampling == PPM_SAMPLE_GL ? g.n_gl : 0};
+  auto it = BS->classes.find(key);
+  if (it == BS->classes.end()) {
+    if (BS->classes.size() >= kMaxClassGrids) {
+      auto victim = BS->classes.end();
+      for (auto v = BS->classes.begin(); v != BS->classes.end(); ++v)
+        if (v->second.last_use < launch_clock &&
+            (victim == BS->classes.end() || v->second.last_use < victim->second.last_use))
+          victim = v;
+      if (victim != BS->classes.end()) {
+        victim->second.release();
+        BS->classes.erase(victim);
+      }
+    }
+    ClassCache& CC = BS->classes[key];
+    build_classes(g.n_side, pts, CC.tab);
+    const ClassTables& T = CC.tab;
+    int e;
+    if ((e = CC.csets.ensure(sizeof(ClassSet) * T.sets.size())) ||
+        (e = CC.chdr.ensure(sizeof(ClassHdr) * std::max<size_t>(T.hdr.size(), 1))) ||
+        (e = CC.cpts.ensure(sizeof(ClassPt) * std::max<size_t>(T.pts.size(), 1))) ||
+        (e = hipMemcpy(CC.csets.p, T.sets.data(), sizeof(ClassSet) * T.sets.size(), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(CC.chdr.p, T.hdr.data(), sizeof(ClassHdr) * T.hdr.size(), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(CC.cpts.p, T.pts.data(), sizeof(ClassPt) * T.pts.size(), hipMemcpyHostToDevice))) {
+      CC.release();
+      BS->classes.erase(key);
+      return hip_fail(c, e, "class tables");
+    }
+    it = BS->classes.find(key);
+  }
+  it->second.last_use = ++BS->use_clock;
+  out = &it->second;
+  return PPM_OK;
+}
+
+constexpr int kEvCap = 512;  // events per class group in LDS
+
+// LDS layout of a launch over `groups`: the largest fine table, slice rows
+// and coverage bitmap among them.  Returns the LDS bytes per workgroup.
+// Energies per lane of a launch: 2 when every grid has at most 128
+// energies, else 6.
+static int launch_epl(const std::vector<BatchGroup>& groups) {
+  size_t ne = 0;
+  for (const BatchGroup& G : groups) ne = std::max(ne, G.energies.size());
+  return ne <= 128 ? 2 : 6;
+}
+
+static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& groups, int atm, int rows_override,
+                              int& nsx_rows, int& nf_max, int& cov_words) {
+  const int per_item = batch_energies(launch_epl(groups));
+  nsx_rows = nf_max = cov_words = 0;
+  const double h = (c->N.logE_max - c->N.logE_min) / std::max(c->N.nE - 1, 1);
+  for (const BatchGroup& G : groups) {
+    nf_max = std::max(nf_max, G.g.n_fine);
+    cov_words = std::max(cov_words, (4 * G.g.n_side + 63) / 64 * 2 + 2);
+    if (atm == PPM_ATM_NSX && !rows_override) {
+      double span = 0;
+      const int ne = (int)G.energies.size();
+      for (int e0 = 0; e0 < ne; e0 += per_item) {
+        const int e1 = std::min(ne, e0 + per_item) - 1;
+        span = std::max(span, std::log10(G.energies[e1] / G.energies[e0]) / h);
+      }
+      nsx_rows = std::max(nsx_rows, std::min(c->N.nE, (int)std::ceil(span) + 16 + 8));
+    }
+  }
+  if (atm == PPM_ATM_NSX) {
+    if (rows_override) nsx_rows = rows_override;
+    nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
+  }
+  return batch_lds_bytes(nf_max, atm, nsx_rows, cov_words, kEvCap);
+}
+
+bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups) {
+  if (groups.empty() || !groups[0].n) return false;
+  int r, f, w;
+  return launch_geometry(c, groups, groups[0].stars[0].atmosphere, 0, r, f, w) <= (size_t)c->max_lds;
+}
+
+static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats, int rows_override) {
   BatchState* BS = state(c);
   BatchWorkspace& W = c->batch;
-  const int n_e = (int)energies.size();
-  const int nf1 = g.n_fine + 1;
-  for (int e = 1; e < n_e; ++e)
-    if (!(energies[e] > energies[e - 1]))
-      return fail(c, PPM_ERR_ARG, "the batched path needs strictly increasing energies");
-  int atm = stars[0].atmosphere;
-  for (int i = 1; i < n; ++i)
-    if (stars[i].atmosphere != atm) return fail(c, PPM_ERR_ARG, "a batch needs one atmosphere model");
+  const int n_groups = (int)groups.size();
+  int n = 0;  // stars of the launch
+  for (const BatchGroup& G : groups) n += G.n;
+  if (!n_groups || n < 1) return fail(c, PPM_ERR_ARG, "no stars");
+  const int atm = groups[0].stars[0].atmosphere;
+  for (const BatchGroup& G : groups) {
+    for (size_t e = 1; e < G.energies.size(); ++e)
+      if (!(G.energies[e] > G.energies[e - 1]))
+        return fail(c, PPM_ERR_ARG, "the batched path needs strictly increasing energies");
+    for (int i = 0; i < G.n; ++i)
+      if (G.stars[i].atmosphere != atm) return fail(c, PPM_ERR_ARG, "a batch needs one atmosphere model");
+  }
   int e;
   if (!BS->n_cu && hipDeviceGetAttribute(&BS->n_cu, hipDeviceAttributeMultiprocessorCount, c->device))
     BS->n_cu = 256;
+  const int epl = launch_epl(groups);
+  const int per_item = batch_energies(epl);
 
   // ---- class tables (per grid)
-  ClassCache& CC = BS->classes;
-  if (CC.n_side != g.n_side || CC.n_phase != g.n_phase || CC.sampling != g.sampling || CC.n_gl != g.n_gl) {
-    build_classes(g.n_side, pts, CC.tab);
-    CC.n_side = g.n_side;
-    CC.n_phase = g.n_phase;
-    CC.sampling = g.sampling;
-    CC.n_gl = g.n_gl;
-    CC.uploaded = false;
-  }
-  if (!CC.uploaded) {
-    const ClassTables& T = CC.tab;
-    if ((e = BS->d_csets.ensure(sizeof(ClassSet) * T.sets.size())) ||
-        (e = BS->d_chdr.ensure(sizeof(ClassHdr) * T.hdr.size())) ||
-        (e = BS->d_cpts.ensure(sizeof(ClassPt) * T.pts.size())) ||
-        (e = hipMemcpy(BS->d_csets.p, T.sets.data(), sizeof(ClassSet) * T.sets.size(), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(BS->d_chdr.p, T.hdr.data(), sizeof(ClassHdr) * T.hdr.size(), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(BS->d_cpts.p, T.pts.data(), sizeof(ClassPt) * T.pts.size(), hipMemcpyHostToDevice)))
-      return hip_fail(c, e, "class tables");
-    CC.uploaded = true;
+  const uint64_t launch_clock = BS->use_clock + 1;
+  std::vector<ClassCache*> cls(n_groups);
+  for (int k = 0; k < n_groups; ++k) {
+    ppm_status s = class_tables(c, BS, groups[k].g, groups[k].pts, launch_clock, cls[k]);
+    if (s) return s;
   }
 
-  // ---- spots, ring tasks and cost estimates
+  // ---- spots, ring tasks and cost estimates; the tasks of a grid are
+  // contiguous, stars are numbered across the grids in order
   std::vector<SpotParams> sp;
   std::vector<RingTask> tasks;
   std::vector<double> cost;
-  std::vector<int> spot_first_task, spot_star;
+  std::vector<int> spot_first_task, spot_star, spot_group;
+  std::vector<int> group_task0(n_groups + 1), group_star0(n_groups + 1);
   std::vector<int> rings;
-  for (int s = 0; s < n; ++s)
-    for (int k = 0; k < stars[s].n_spots; ++k) {
-      SpotParams P;
-      spot_params(c, stars[s], stars[s].spot[k], g.n_side, P);
-      P.pad = s;
-      const int si = (int)sp.size();
-      sp.push_back(P);
-      spot_first_task.push_back((int)tasks.size());
-      spot_star.push_back(s);
-      candidate_rings(P, stars[s].spot[k], g.n_side, rings);
-      const ppm_spot& spt = stars[s].spot[k];
-      for (int r : rings) {
-        tasks.push_back(RingTask{si, r});
-        double est;
-        if (r == 0) {
-          est = (double)pts.size();
-        } else {
-          const int npix = 4 * std::min({r, 4 * g.n_side - r, g.n_side});
-          const ClassSet cs = CC.tab.sets[npix / 4];
-          const double a = arc_fraction(std::acos(ring_z(g.n_side, r)), spt.theta1, spt.angrad1) * npix;
-          const double per_cls = (double)pts.size() / std::max(cs.n_classes, 1);
-          est = cs.n_classes * (std::min((double)npix, a * per_cls) + 64.0);
+  for (int k = 0, sg = 0; k < n_groups; ++k) {
+    const BatchGroup& G = groups[k];
+    const ppm_grid& g = G.g;
+    group_task0[k] = (int)tasks.size();
+    group_star0[k] = sg;
+    for (int s = 0; s < G.n; ++s, ++sg)
+      for (int q = 0; q < G.stars[s].n_spots; ++q) {
+        SpotParams P;
+        spot_params(c, G.stars[s], G.stars[s].spot[q], g.n_side, P);
+        P.pad = sg;
+        const int si = (int)sp.size();
+        sp.push_back(P);
+        spot_first_task.push_back((int)tasks.size());
+        spot_star.push_back(sg);
+        spot_group.push_back(k);
+        candidate_rings(P, G.stars[s].spot[q], g.n_side, rings);
+        const ppm_spot& spt = G.stars[s].spot[q];
+        for (int r : rings) {
+          tasks.push_back(RingTask{si, r});
+          double est;
+          if (r == 0) {
+            est = (double)G.pts.size();
+          } else {
+            const int npix = 4 * std::min({r, 4 * g.n_side - r, g.n_side});
+            const ClassSet cs = cls[k]->tab.sets[npix / 4];
+            const double a = arc_fraction(std::acos(ring_z(g.n_side, r)), spt.theta1, spt.angrad1) * npix;
+            const double per_cls = (double)G.pts.size() / std::max(cs.n_classes, 1);
+            est = cs.n_classes * (std::min((double)npix, a * per_cls) + 64.0);
+          }
+          // an energy chunk of 384 energies costs about as much as the full
+          // chunk; fewer energies cost proportionally less
+          cost.push_back(est * std::min(1.0, (double)G.energies.size() / per_item));
         }
-        cost.push_back(est);
       }
-    }
+  }
+  group_task0[n_groups] = (int)tasks.size();
+  group_star0[n_groups] = n;
   spot_first_task.push_back((int)tasks.size());
   const int n_spots = (int)sp.size();
   const int n_tasks = (int)tasks.size();
 
-  // ---- launch geometry
-  const int n_echunk = (n_e + batch_energies() - 1) / batch_energies();
-  int nsx_rows = 0;
-  if (atm == PPM_ATM_NSX) {
-    if (rows_override) {
-      nsx_rows = rows_override;
-    } else {
-      const double h = (c->N.logE_max - c->N.logE_min) / (c->N.nE - 1);
-      double span = 0;
-      for (int e0 = 0; e0 < n_e; e0 += batch_energies()) {
-        const int e1 = std::min(n_e, e0 + batch_energies()) - 1;
-        span = std::max(span, std::log10(energies[e1] / energies[e0]) / h);
-      }
-      nsx_rows = std::min(c->N.nE, (int)std::ceil(span) + 16 + 8);
-    }
-    nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
-  }
-  const int cov_words = (4 * g.n_side + 63) / 64 * 2 + 2;
-  const int ev_cap = 512;
-  const size_t lds = batch_lds_bytes(g.n_fine, atm, nsx_rows, cov_words, ev_cap);
+  // ---- launch geometry: the LDS layout of the largest grid
+  int nsx_rows, nf_max, cov_words;
+  const size_t lds = launch_geometry(c, groups, atm, rows_override, nsx_rows, nf_max, cov_words);
   if (lds > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
-  if ((e = batched_prepare(atm, lds))) return hip_fail(c, e, "kernel attributes");
+  if ((e = batched_prepare(atm, epl, lds))) return hip_fail(c, e, "kernel attributes");
   int wg_per_cu = 0;
-  if (batched_occupancy(atm, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
+  if (batched_occupancy(atm, epl, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
   BS->occ_wg_max = wg_per_cu;
   ppm_launch_shape shape{0, 0};
   if (BS->forced) {
     shape = *BS->forced;
-  } else {
-    auto it = BS->shapes.find(shape_key(g, n_e, atm, n));
+  } else if (n_groups == 1) {
+    auto it = BS->shapes.find(shape_key(groups[0].g, (int)groups[0].energies.size(), atm, n));
     if (it != BS->shapes.end()) shape = it->second;
   }
   if (shape.wg_per_cu > 0) wg_per_cu = std::min(wg_per_cu, shape.wg_per_cu);
   const int slots = wg_per_cu * BS->n_cu;
 
-  // ---- items: each spot's rings split into chunks of similar cost
+  // ---- items: each spot's rings split into chunks of similar cost, times
+  // the energy chunks of its grid; about per_slot items per resident
+  // workgroup, fewer if their accumulators would pass kAccCapBytes
+  std::vector<int> n_echunk(n_groups);
+  std::vector<size_t> acc_per_item(n_groups);
+  for (int k = 0; k < n_groups; ++k) {
+    const int ne = (int)groups[k].energies.size();
+    n_echunk[k] = (ne + per_item - 1) / per_item;
+    acc_per_item[k] = (size_t)groups[k].g.n_phase * ne * batch_acc_copies();
+  }
   double total_cost = 0;
   for (double v : cost) total_cost += v;
-  const size_t acc_per_item = (size_t)g.n_phase * n_e * batch_acc_copies();
-  const size_t cap_items = std::max<size_t>(1, ((size_t)3072 << 20) / (acc_per_item * sizeof(double)));
   const double per_slot = shape.items_per_slot > 0 ? shape.items_per_slot : kDefaultItemsPerSlot;
-  const double target = std::min<double>((double)cap_items / n_echunk, per_slot * slots / n_echunk);
   std::vector<BatchItem> items;
   std::vector<double> item_cost;
   std::vector<int> star_item0(n + 1, 0);
-  for (int s = 0, si = 0; s < n; ++s) {
-    star_item0[s] = (int)items.size();
-    for (; si < n_spots && spot_star[si] == s; ++si) {
-      const int t0 = spot_first_task[si], t1 = spot_first_task[si + 1];
-      if (t0 == t1) continue;
-      double cs = 0;
-      for (int t = t0; t < t1; ++t) cs += cost[t];
-      int k = (int)std::lround(total_cost > 0 ? target * cs / total_cost : 1.0);
-      k = std::max(1, std::min(k, t1 - t0));
-      double acc = 0;
-      int start = t0, made = 0;
-      for (int t = t0; t < t1; ++t) {
-        acc += cost[t];
-        if (t == t1 - 1 || acc >= cs * (made + 1) / k) {
-          double ic = 0;
-          for (int u = start; u <= t; ++u) ic += cost[u];
-          for (int ec = 0; ec < n_echunk; ++ec) {
-            items.push_back(BatchItem{start, t + 1, s, si, ec * batch_energies(), 0});
-            item_cost.push_back(ic);
+  size_t acc_total = 0;
+  auto build_items = [&](double target) {
+    items.clear();
+    item_cost.clear();
+    acc_total = 0;
+    for (int s = 0, si = 0; s < n; ++s) {
+      star_item0[s] = (int)items.size();
+      for (; si < n_spots && spot_star[si] == s; ++si) {
+        const int t0 = spot_first_task[si], t1 = spot_first_task[si + 1];
+        if (t0 == t1) continue;
+        const int k_grid = spot_group[si];
+        double cs = 0;
+        for (int t = t0; t < t1; ++t) cs += cost[t];
+        int k = (int)std::lround(total_cost > 0 ? target * cs / total_cost / n_echunk[k_grid] : 1.0);
+        k = std::max(1, std::min(k, t1 - t0));
+        double acc = 0;
+        int start = t0, made = 0;
+        for (int t = t0; t < t1; ++t) {
+          acc += cost[t];
+          if (t == t1 - 1 || acc >= cs * (made + 1) / k) {
+            double ic = 0;
+            for (int u = start; u <= t; ++u) ic += cost[u];
+            for (int ec = 0; ec < n_echunk[k_grid]; ++ec) {
+              items.push_back(BatchItem{start, t + 1, s, si, ec * per_item, k_grid, acc_total});
+              acc_total += acc_per_item[k_grid];
+              item_cost.push_back(ic);
+            }
+            start = t + 1;
+            ++made;
           }
-          start = t + 1;
-          ++made;
         }
       }
     }
-  }
+  };
+  const double target = per_slot * slots;
+  build_items(target);
+  if (acc_total * sizeof(double) > kAccCapBytes)
+    build_items(std::max(1.0, target * (double)kAccCapBytes / (double)(acc_total * sizeof(double))));
   star_item0[n] = (int)items.size();
   const int n_items = (int)items.size();
   std::vector<int> order(n_items);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return item_cost[a] > item_cost[b]; });
 
+  // ---- per-grid offsets of energies and fine tables
+  std::vector<size_t> e_off(n_groups), ph_off(n_groups), fine_off(n_groups);
+  size_t n_energy_all = 0, n_ph_all = 0, n_fine_all = 0;
+  for (int k = 0; k < n_groups; ++k) {
+    const size_t nt = group_task0[k + 1] - group_task0[k], nf1 = groups[k].g.n_fine + 1;
+    e_off[k] = n_energy_all;
+    n_energy_all += groups[k].energies.size();
+    ph_off[k] = n_ph_all;
+    n_ph_all += nt * nf1;
+    fine_off[k] = n_fine_all;
+    n_fine_all += 3 * nt * nf1;
+  }
+
   // ---- buffers and uploads
   if ((e = W.spots.ensure(sizeof(SpotParams) * std::max(n_spots, 1))) ||
       (e = W.tasks.ensure(sizeof(RingTask) * std::max(n_tasks, 1))) ||
       (e = W.desc.ensure(sizeof(RingDesc) * std::max(n_tasks, 1))) ||
-      (e = W.ph.ensure(sizeof(double) * nf1 * std::max(n_tasks, 1))) ||
-      (e = W.fine.ensure(sizeof(float) * 3 * nf1 * std::max(n_tasks, 1))) ||
-      (e = W.energies.ensure(sizeof(double) * n_e)) || (e = W.items.ensure(sizeof(BatchItem) * std::max(n_items, 1))) ||
+      (e = W.ph.ensure(sizeof(double) * std::max<size_t>(n_ph_all, 1))) ||
+      (e = W.fine.ensure(sizeof(float) * std::max<size_t>(n_fine_all, 1))) ||
+      (e = W.energies.ensure(sizeof(double) * n_energy_all)) ||
+      (e = W.items.ensure(sizeof(BatchItem) * std::max(n_items, 1))) ||
+      (e = BS->d_grids.ensure(sizeof(BatchGrid) * n_groups)) ||
       (e = BS->d_order.ensure(sizeof(int) * std::max(n_items, 1))) ||
-      (e = BS->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) || (e = W.evals.ensure(sizeof(unsigned long long))) ||
-      (e = BS->d_acc.ensure(sizeof(double) * acc_per_item * std::max(n_items, 1))) ||
+      (e = BS->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) ||
+      (e = W.evals.ensure(sizeof(unsigned long long))) ||
+      (e = BS->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
       (e = BS->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
     return hip_fail(c, e, "device allocation");
-  std::vector<double> fold_scale(n);
-  for (int s = 0; s < n; ++s) fold_scale[s] = stars[s].exposure_s * (1.0 / g.n_phase);
+  std::vector<BatchGrid> bg(n_groups);
+  std::vector<double> energies_all(n_energy_all), fold_scale(n);
+  for (int k = 0; k < n_groups; ++k) {
+    const BatchGroup& G = groups[k];
+    std::copy(G.energies.begin(), G.energies.end(), energies_all.begin() + e_off[k]);
+    bg[k] = BatchGrid{(const double*)W.energies.p + e_off[k],
+                      (const ClassSet*)cls[k]->csets.p,
+                      (const ClassHdr*)cls[k]->chdr.p,
+                      (const ClassPt*)cls[k]->cpts.p,
+                      (const double*)W.ph.p + ph_off[k],
+                      (const float*)W.fine.p + fine_off[k],
+                      (int)G.energies.size(),
+                      G.g.n_phase,
+                      G.g.n_fine,
+                      group_task0[k]};
+    for (int s = 0; s < G.n; ++s)
+      fold_scale[group_star0[k] + s] = G.stars[s].exposure_s * (1.0 / G.g.n_phase);
+  }
   {
     Upload up;
     up.add(W.spots.p, sp.data(), sizeof(SpotParams) * n_spots);
     if (n_tasks) up.add(W.tasks.p, tasks.data(), sizeof(RingTask) * n_tasks);
-    up.add(W.energies.p, energies.data(), sizeof(double) * n_e);
+    up.add(W.energies.p, energies_all.data(), sizeof(double) * n_energy_all);
+    up.add(BS->d_grids.p, bg.data(), sizeof(BatchGrid) * n_groups);
     if (n_items) {
       up.add(W.items.p, items.data(), sizeof(BatchItem) * n_items);
       up.add(BS->d_order.p, order.data(), sizeof(int) * n_items);
@@ -332,24 +476,24 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
     if ((e = up.run(c))) return hip_fail(c, e, "upload");
   }
 
-  PlainLaunch P{};
-  P.d_spots = (const SpotParams*)W.spots.p;
-  P.d_tasks = (const RingTask*)W.tasks.p;
-  P.d_desc = (RingDesc*)W.desc.p;
-  P.d_ph = (double*)W.ph.p;
-  P.d_fine = (float*)W.fine.p;
-  P.n_tasks = n_tasks;
-  P.n_fine = g.n_fine;
+  // ring set-up per grid (its fine-table stride)
+  std::vector<PlainLaunch> P(n_groups);
+  for (int k = 0; k < n_groups; ++k) {
+    PlainLaunch& Q = P[k];
+    Q = PlainLaunch{};
+    Q.d_spots = (const SpotParams*)W.spots.p;
+    Q.d_tasks = (const RingTask*)W.tasks.p + group_task0[k];
+    Q.d_desc = (RingDesc*)W.desc.p + group_task0[k];
+    Q.d_ph = (double*)W.ph.p + ph_off[k];
+    Q.d_fine = (float*)W.fine.p + fine_off[k];
+    Q.n_tasks = group_task0[k + 1] - group_task0[k];
+    Q.n_fine = groups[k].g.n_fine;
+  }
   BatchLaunch B{};
-  B.spots = P.d_spots;
-  B.tasks = P.d_tasks;
-  B.desc = P.d_desc;
-  B.ph = P.d_ph;
-  B.fine = P.d_fine;
-  B.energies = (const double*)W.energies.p;
-  B.csets = (const ClassSet*)BS->d_csets.p;
-  B.chdr = (const ClassHdr*)BS->d_chdr.p;
-  B.cpts = (const ClassPt*)BS->d_cpts.p;
+  B.spots = (const SpotParams*)W.spots.p;
+  B.tasks = (const RingTask*)W.tasks.p;
+  B.desc = (RingDesc*)W.desc.p;
+  B.grids = (const BatchGrid*)BS->d_grids.p;
   B.items = (const BatchItem*)W.items.p;
   B.order = (const int*)BS->d_order.p;
   B.star_item0 = (const int*)BS->d_star_item0.p;
@@ -357,59 +501,64 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
   B.n_evals = (unsigned long long*)W.evals.p;
   B.acc = (double*)BS->d_acc.p;
   B.seg = (double*)BS->d_seg.p;
-  B.flux = d_flux;
   B.n_items = n_items;
-  B.n_stars = n;
-  B.n_e = n_e;
-  B.n_phase = g.n_phase;
-  B.n_fine = g.n_fine;
+  B.n_fine_max = nf_max;
   B.nsx_rows = nsx_rows;
   B.cov_words = cov_words;
-  B.ev_cap = ev_cap;
+  B.ev_cap = kEvCap;
+  B.epl = epl;
   B.atmosphere = atm;
   const int n_wg = std::max(1, std::min(slots, n_items));
-  const bool fold = d_counts && g.response >= 0;
 
   // ---- the launch sequence, captured once per configuration and replayed
-  std::vector<unsigned char> key(sizeof(P) + sizeof(B) + 4 * sizeof(int) + sizeof(void*));
-  {
-    unsigned char* k = key.data();
-    std::memcpy(k, &P, sizeof P);
-    k += sizeof P;
-    std::memcpy(k, &B, sizeof B);
-    k += sizeof B;
-    const int extra[4] = {n_wg, fold ? g.response : -1, (int)lds, 0};
-    std::memcpy(k, extra, sizeof extra);
-    k += sizeof extra;
-    std::memcpy(k, &d_counts, sizeof(void*));
+  std::vector<unsigned char> key;
+  auto put = [&](const void* p, size_t nb) {
+    const unsigned char* b = (const unsigned char*)p;
+    key.insert(key.end(), b, b + nb);
+  };
+  put(&B, sizeof B);
+  put(&n_wg, sizeof n_wg);
+  put(&lds, sizeof lds);
+  for (int k = 0; k < n_groups; ++k) {
+    put(&P[k], sizeof(PlainLaunch));
+    put(&bg[k], sizeof(BatchGrid));
+    put(&groups[k].d_flux, sizeof(void*));
+    put(&groups[k].d_counts, sizeof(void*));
+    put(&groups[k].g.response, sizeof(int));
+    put(&group_star0[k], sizeof(int));
   }
   // timed: per-kernel events around a direct launch; graph replays are timed
   // as a whole by events recorded around hipGraphLaunch (outside the graph)
   auto record = [&](bool timed) -> int {
     int r;
     if (timed && (r = hipEventRecord(c->ev0, c->stream))) return r;
-    if ((r = launch_ring_setup(c->L, c->N, P, c->stream))) return r;
+    for (int k = 0; k < n_groups; ++k)
+      if (P[k].n_tasks && (r = launch_ring_setup(c->L, c->N, P[k], c->stream))) return r;
     if (timed && (r = hipEventRecord(c->ev_i0, c->stream))) return r;
     if (n_items && (r = launch_batched(c->N, B, n_wg, c->stream))) return r;
     if (timed && (r = hipEventRecord(c->ev_i1, c->stream))) return r;
-    if ((r = launch_reduce_batched(B, c->stream))) return r;
+    for (int k = 0; k < n_groups; ++k)
+      if ((r = launch_reduce_batched(B, bg[k], group_star0[k], group_star0[k + 1], groups[k].d_flux, c->stream)))
+        return r;
     if (timed && (r = hipEventRecord(c->ev1, c->stream))) return r;
-    if (fold) {
-      const Response& R = c->resp[g.response];
-      if ((r = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, n, g.n_phase, (const double*)W.scale.p, d_counts,
-                           c->stream)))
+    for (int k = 0; k < n_groups; ++k) {
+      const BatchGroup& G = groups[k];
+      if (!G.d_counts || G.g.response < 0) continue;
+      const Response& R = c->resp[G.g.response];
+      if ((r = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, G.d_flux, G.n, G.g.n_phase,
+                           (const double*)W.scale.p + group_star0[k], G.d_counts, c->stream)))
         return r;
     }
     if (timed && (r = hipEventRecord(c->ev2, c->stream))) return r;
     return hipSuccess;
   };
-  GraphCache& G = BS->graph;
-  const bool use_graph = !c->no_graph && ((G.exec && G.key == key) || G.last_direct == key);
+  GraphCache& GC = BS->graph;
+  const bool use_graph = !c->no_graph && ((GC.exec && GC.key == key) || GC.last_direct == key);
   if (use_graph) {
-    if (!G.exec || G.key != key) {
-      if (G.exec) {
-        hipGraphExecDestroy(G.exec);
-        G.exec = nullptr;
+    if (!GC.exec || GC.key != key) {
+      if (GC.exec) {
+        hipGraphExecDestroy(GC.exec);
+        GC.exec = nullptr;
       }
       hipGraph_t graph = nullptr;
       if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal))) return hip_fail(c, e, "capture");
@@ -420,20 +569,20 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
         return hip_fail(c, r, "batched launch");
       }
       if (e) return hip_fail(c, e, "capture");
-      e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&GC.exec, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e) {
-        G.exec = nullptr;
+        GC.exec = nullptr;
         return hip_fail(c, e, "graph instantiation");
       }
-      G.key = key;
+      GC.key = key;
     }
     if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");
-    if ((e = hipGraphLaunch(G.exec, c->stream))) return hip_fail(c, e, "graph launch");
+    if ((e = hipGraphLaunch(GC.exec, c->stream))) return hip_fail(c, e, "graph launch");
     if ((e = hipEventRecord(c->ev2, c->stream))) return hip_fail(c, e, "event");
   } else {
     if ((e = record(true))) return hip_fail(c, e, "batched launch");
-    G.last_direct = key;
+    GC.last_direct = key;
   }
   if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
 
@@ -442,15 +591,16 @@ ppm_status eval_batched_rows(ppm_ctx* c, const ppm_star* stars, int n, const ppm
     return hip_fail(c, e, "descriptor read-back");
   bool rows_overflow = false;
   for (const RingDesc& d : desc) rows_overflow |= (d.flags & 2) != 0;
-  if (rows_overflow && nsx_rows < c->N.nE)
-    return eval_batched_rows(c, stars, n, g, energies, pts, d_flux, d_counts, stats, c->N.nE);
+  if (rows_overflow && nsx_rows < c->N.nE) return eval_groups_rows(c, groups, stats, c->N.nE);
   int64_t n_act = 0, n_eval_ref = 0, n_rings = 0;
-  for (const RingDesc& d : desc) {
-    if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
-    n_act += d.n_act;
-    if (d.n_act) ++n_rings;
-    if (d.kept) n_eval_ref += (int64_t)d.n_act * (int64_t)pts.size() * n_e;
-  }
+  for (int k = 0; k < n_groups; ++k)
+    for (int t = group_task0[k]; t < group_task0[k + 1]; ++t) {
+      const RingDesc& d = desc[t];
+      if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
+      n_act += d.n_act;
+      if (d.n_act) ++n_rings;
+      if (d.kept) n_eval_ref += (int64_t)d.n_act * (int64_t)groups[k].pts.size() * groups[k].energies.size();
+    }
   if (stats) {
     float t02 = 0;
     hipEventElapsedTime(&t02, c->ev0, c->ev2);

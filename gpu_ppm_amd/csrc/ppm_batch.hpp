@@ -11,4 +11,22 @@ ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats);
 
+// Stars on one grid within a launch that may hold several grids.
+struct BatchGroup {
+  const ppm_star* stars = nullptr;
+  int n = 0;
+  ppm_grid g{};                   // validated; its energies are `energies`
+  std::vector<double> energies;   // ascending
+  std::vector<PhasePoint> pts;
+  double* d_flux = nullptr;       // [n][n_phase][n_e]
+  double* d_counts = nullptr;     // [n][n_ch][n_phase], or null (no fold)
+};
+
+// One persistent launch over the stars of several grids (one atmosphere,
+// energies strictly increasing in every group).
+ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats);
+
+// Whether one launch over these groups fits the LDS of a workgroup.
+bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups);
+
 }  // namespace ppm

@@ -25,45 +25,55 @@ struct ClassPt {
   double w;  // phase-point weight
 };
 
+// Per-grid facts of a batched launch: one launch can hold stars on several
+// grids (the serving loop packs every queued request into one launch).
+struct BatchGrid {
+  const double* energies;  // [n_e], ascending
+  const ClassSet* csets;   // class tables of the grid's phase points
+  const ClassHdr* chdr;
+  const ClassPt* cpts;
+  const double* ph;        // fine phase tables of the grid's ring tasks, [task - task0][nf1]
+  const float* fine;       // [task - task0][3][nf1]
+  int n_e, n_phase, n_fine, task0;
+};
+
 // A work item: contiguous ring tasks of one spot of one star, and one chunk
 // of the energy axis.
 struct BatchItem {
   int task0, task1;
   int star, spot;
-  int e0;     // first energy of the chunk
-  int pad;
+  int e0;      // first energy of the chunk
+  int grid;    // index into BatchLaunch::grids
+  size_t acc;  // first double of the item's accumulator copies, [copies][n_phase][n_e]
 };
 
 struct BatchLaunch {
   const SpotParams* spots;
   const RingTask* tasks;
   RingDesc* desc;
-  const double* ph;
-  const float* fine;
-  const double* energies;
-  const ClassSet* csets;
-  const ClassHdr* chdr;
-  const ClassPt* cpts;
+  const BatchGrid* grids;
   const BatchItem* items;
   const int* order;        // queue order of the items (largest first)
   const int* star_item0;   // [n_stars + 1] item ranges per star
   int* queue;
   unsigned long long* n_evals;  // (position, energy) evaluations performed
-  double* acc;             // [n_items][copies][n_phase][n_e]
+  double* acc;             // item accumulators
   double* seg;             // [n_wg][segment totals], workgroup scratch
-  double* flux;            // [n_stars][n_phase][n_e]
-  int n_items, n_stars, n_e, n_phase, n_fine;
+  int n_items;
+  int n_fine_max;          // LDS layout: the largest grid of the launch
   int nsx_rows, cov_words, ev_cap, atmosphere;
+  int epl;                 // energies per lane: 6, or 2 when every grid has <= 128 energies
 };
 
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals
-int batch_energies();
-int batched_prepare(int atmosphere, size_t lds);  // outside any stream capture
+int batch_energies(int epl);  // energies per item
+int batched_prepare(int atmosphere, int epl, size_t lds);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);
-int launch_reduce_batched(const BatchLaunch& B, void* stream);
-int batched_occupancy(int atmosphere, size_t lds, int* wg_per_cu);
+// flux[s - s0][b][e] for the stars [s0, s1) of one grid (G, its host copy).
+int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream);
+int batched_occupancy(int atmosphere, int epl, size_t lds, int* wg_per_cu);
 
 }  // namespace ppm

@@ -25,9 +25,10 @@ namespace {
 constexpr int kBW = 4;                  // wavefronts per workgroup, each a segment of the positions
 constexpr int kBWP = kBW;
 constexpr int kBThreads = 64 * kBW;
-constexpr int kBEPL = 6;                // energies per lane
-constexpr int kBSets = 2;               // row sets of a lane's energies (3 u each)
-constexpr int kBEnergies = 64 * kBEPL;  // energies per workgroup (384)
+// Energies per lane: 6 (384 per item, energy grids beyond 128 points) or
+// 2 (128 per item); a lane's energies form two row sets either way.
+constexpr int kMaxEPL = 6;
+constexpr int kBSets = 2;
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
 // pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096).
@@ -124,14 +125,15 @@ __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
   }
 }
 
-template <int ATM>
+template <int ATM, int EPL>
 __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, BatchLaunch B) {
+  constexpr int kBEPL = EPL;
+  constexpr int kBEnergies = 64 * EPL;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wp = wave;  // position part
-  const int nf1 = B.n_fine + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rows_alloc = ATM == PPM_ATM_NSX ? B.nsx_rows : 0;
-  const BLds lay = batch_layout(nf1, rows_alloc, B.cov_words, B.ev_cap);
+  const BLds lay = batch_layout(B.n_fine_max + 1, rows_alloc, B.cov_words, B.ev_cap);
   double* s_ph = reinterpret_cast<double*>(smem + lay.ph);
   double* s_mu = reinterpret_cast<double*>(smem + lay.mu);
   float* s_fI = reinterpret_cast<float*>(smem + lay.fI);
@@ -148,7 +150,6 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   if (ATM == PPM_ATM_NSX)
     for (int k = tid; k < N.nm; k += kBThreads) s_mu[k] = N.mu[k];
 
-  const int n_e = B.n_e;
   const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
   const double inv_h = 1.0 / h;
   unsigned int n_done = 0;  // evaluations by this lane
@@ -160,17 +161,20 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
     const int q_idx = s_item;
     if (q_idx >= B.n_items) break;
     const BatchItem It = B.items[B.order[q_idx]];
+    const BatchGrid G = B.grids[It.grid];
+    const int n_e = G.n_e, n_phase = G.n_phase, nf1 = G.n_fine + 1;
+    const double* energies = G.energies;
     // one accumulator copy per position part: each address has one writer;
     // acc points at the item's first energy
-    double* acc = B.acc + ((size_t)B.order[q_idx] * kBWP + wp) * B.n_phase * n_e + It.e0;
+    double* acc = B.acc + It.acc + (size_t)wp * n_phase * n_e + It.e0;
     // energies of this thread: e = e0 + u*64 + lane (ascending grid), valid
     // while u*64 + lane <= e_lim
     const int e_hi = min(It.e0 + kBEnergies, n_e) - 1, e_lim = e_hi - It.e0;
-    const double l_ref = log10(B.energies[It.e0]);
+    const double l_ref = log10(energies[It.e0]);
     float le[kBEPL], Ef[kBEPL];
 #pragma unroll
     for (int u = 0; u < kBEPL; ++u) {
-      const double E = B.energies[It.e0 + min(u * 64 + lane, e_lim)];
+      const double E = energies[It.e0 + min(u * 64 + lane, e_lim)];
       Ef[u] = (float)E;
       le[u] = (float)((log10(E) - l_ref) * inv_h);
     }
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
 #pragma unroll
     for (int u = 0; u < kBEPL; ++u) n_valid += u * 64 + lane <= e_lim;
     const WaveEnergies<kBSets> we = wave_energies<kBEPL, kBSets>(le, lane);
-    for (int b = 0; b < B.n_phase; ++b)
+    for (int b = 0; b < n_phase; ++b)
 #pragma unroll
       for (int u = 0; u < kBEPL; ++u)
         if (u * 64 + lane <= e_lim) acc[(size_t)b * n_e + u * 64 + lane] = 0.0;
@@ -191,8 +195,8 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const SpotParams& S = B.spots[T.spot];
       __syncthreads();  // previous ring's LDS is no longer read
       {
-        const double* gph = B.ph + (size_t)task * nf1;
-        const float* gf = B.fine + (size_t)task * 3 * nf1;
+        const double* gph = G.ph + (size_t)(task - G.task0) * nf1;
+        const float* gf = G.fine + (size_t)(task - G.task0) * 3 * nf1;
         for (int k = tid; k < nf1; k += kBThreads) {
           s_ph[k] = gph[k];
           s_fI[k] = gf[k];
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       if (ATM == PPM_ATM_NSX) {
         const double lkT = log10(S.kT_nsx);
         const double t_min = (l_ref + log10(Dp->zf_min) - lkT - N.logE_min) * inv_h;
-        const double t_max = (log10(B.energies[e_hi]) + log10(Dp->zf_max) - lkT - N.logE_min) * inv_h;
+        const double t_max = (log10(energies[e_hi]) + log10(Dp->zf_max) - lkT - N.logE_min) * inv_h;
         r_lo = clampi((int)floor(t_min - 0.01) - 1, 0, N.nE - 4);
         const int r_hi = clampi((int)floor(t_max + 0.01) - 1, 0, N.nE - 4) + 3;
         n_rows = r_hi - r_lo + 1;
@@ -229,12 +233,12 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const int n = point ? 1 : Dp->n_pix;
       const double inv_n = 1.0 / n;
       const int n_runs = point ? 1 : Dp->n_runs;
-      const ClassSet cset = B.csets[point ? 0 : n / 4];
+      const ClassSet cset = G.csets[point ? 0 : n / 4];
       const double c_shift = point ? Dp->ps_shift : 1.0 - 0.25 * Dp->s_shift2;
       const int n_words = (n + 63) / 64 * 2;  // coverage words, 64-position batches
 
       for (int ci = 0; ci < cset.n_classes; ++ci) {
-        const ClassHdr C = B.chdr[cset.first_class + ci];
+        const ClassHdr C = G.chdr[cset.first_class + ci];
         const double c_o = C.delta + c_shift;
         // rotational phase of position k, wrapped into [ph[0], ph[0] + 1)
         auto phase_of = [&](int k) {
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           __syncthreads();
           for (int wi = tid; wi < n_win; wi += kBThreads) {
             const int ql = wi / n_runs, r = wi - ql * n_runs;
-            const ClassPt cp = B.cpts[C.first_pt + g0 + ql];
+            const ClassPt cp = G.cpts[C.first_pt + g0 + ql];
             const int j0 = point ? 1 : Dp->run_start[r];
             const int L = point ? 1 : Dp->run_len[r];
             int a = cp.m + j0 - 1;
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             const uint64_t ki = s_keys[i];
             int rank = 0;
             for (int j = 0; j < n_ev; ++j) rank += s_keys[j] < ki;
-            const ClassPt cp = B.cpts[C.first_pt + g0 + (int)((ki >> 12) & 0xfffff)];
+            const ClassPt cp = G.cpts[C.first_pt + g0 + (int)((ki >> 12) & 0xfffff)];
             s_ev[rank] = EvRec{(uint32_t)(ki >> 32) << 12 | (uint32_t)cp.bin, (float)((ki & 1u) ? cp.w : -cp.w)};
           }
           // ---- keep the covered positions whose patch is seen (flux/I != 0):
@@ -499,21 +503,24 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   if (lane == 0) atomicAdd(B.n_evals, tot);
 }
 
-// flux[s][b][e] = sum over the star's items (in item order) of
-// scale(spot) * (sum of the item's accumulator copies) / E.
-__global__ void k_reduce_batched(BatchLaunch B) {
+// flux[s - s0][b][e] = sum over the star's items (in item order) of
+// scale(spot) * (sum of the item's accumulator copies) / E, for the stars
+// [s0, s1) of one grid.
+__global__ void k_reduce_batched(BatchLaunch B, int s0, int s1, int n_phase, int n_e, const double* energies,
+                                 double* flux) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per = (size_t)B.n_phase * B.n_e;
-  if (idx >= per * B.n_stars) return;
-  const int s = (int)(idx / per);
-  const size_t o = idx - (size_t)s * per;
+  const size_t per = (size_t)n_phase * n_e;
+  if (idx >= per * (s1 - s0)) return;
+  const int s = s0 + (int)(idx / per);
+  const size_t o = idx - (size_t)(s - s0) * per;
   double total = 0.0;
   for (int it = B.star_item0[s]; it < B.star_item0[s + 1]; ++it) {
+    const BatchItem I = B.items[it];
     double a = 0.0;
-    for (int c = 0; c < kBWP; ++c) a += B.acc[((size_t)it * kBWP + c) * per + o];
-    total += B.spots[B.items[it].spot].scale * a;
+    for (int c = 0; c < kBWP; ++c) a += B.acc[I.acc + c * per + o];
+    total += B.spots[I.spot].scale * a;
   }
-  B.flux[idx] = total / B.energies[o % B.n_e];
+  flux[idx] = total / energies[o % n_e];
 }
 
 }  // namespace
@@ -524,38 +531,52 @@ size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, 
 
 int batch_threads() { return kBThreads; }
 int batch_acc_copies() { return kBWP; }
-size_t batch_seg_doubles() { return (size_t)kBWP * kBEPL * 64; }
-int batch_energies() { return kBEnergies; }
+size_t batch_seg_doubles() { return (size_t)kBWP * kMaxEPL * 64; }
+int batch_energies(int epl) { return 64 * epl; }
 
-int batched_prepare(int atmosphere, size_t lds) {
-  const void* f = atmosphere == PPM_ATM_NSX ? (const void*)k_integral_batched<PPM_ATM_NSX>
-                                            : (const void*)k_integral_batched<PPM_ATM_BLACKBODY>;
-  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+namespace {
+const void* batched_fn(int atmosphere, int epl) {
+  if (atmosphere == PPM_ATM_NSX)
+    return epl == 2 ? (const void*)k_integral_batched<PPM_ATM_NSX, 2> : (const void*)k_integral_batched<PPM_ATM_NSX, 6>;
+  return epl == 2 ? (const void*)k_integral_batched<PPM_ATM_BLACKBODY, 2>
+                  : (const void*)k_integral_batched<PPM_ATM_BLACKBODY, 6>;
+}
+}  // namespace
+
+int batched_prepare(int atmosphere, int epl, size_t lds) {
+  return hipFuncSetAttribute(batched_fn(atmosphere, epl), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, B.queue, B.n_evals);
-  const size_t lds = batch_lds_bytes(B.n_fine, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap);
-  if (B.atmosphere == PPM_ATM_NSX)
-    hipLaunchKernelGGL(k_integral_batched<PPM_ATM_NSX>, dim3(n_wg), dim3(kBThreads), lds, st, N, B);
-  else
-    hipLaunchKernelGGL(k_integral_batched<PPM_ATM_BLACKBODY>, dim3(n_wg), dim3(kBThreads), lds, st, N, B);
+  const size_t lds = batch_lds_bytes(B.n_fine_max, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap);
+  const bool nsx = B.atmosphere == PPM_ATM_NSX;
+  if (B.epl == 2) {
+    if (nsx)
+      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_NSX, 2>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
+    else
+      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_BLACKBODY, 2>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
+  } else {
+    if (nsx)
+      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_NSX, 6>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
+    else
+      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_BLACKBODY, 6>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
+  }
   return hipGetLastError();
 }
 
-int launch_reduce_batched(const BatchLaunch& B, void* stream) {
-  const size_t n = (size_t)B.n_phase * B.n_e * B.n_stars;
-  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, B);
+int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream) {
+  const size_t n = (size_t)G.n_phase * G.n_e * (s1 - s0);
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, B, s0, s1,
+                     G.n_phase, G.n_e, G.energies, flux);
   return hipGetLastError();
 }
 
-int batched_occupancy(int atmosphere, size_t lds, int* wg_per_cu) {
+int batched_occupancy(int atmosphere, int epl, size_t lds, int* wg_per_cu) {
   int n = 0;
-  hipError_t e = atmosphere == PPM_ATM_NSX
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_integral_batched<PPM_ATM_NSX>, kBThreads, lds)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_integral_batched<PPM_ATM_BLACKBODY>,
-                                                                    kBThreads, lds);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl), kBThreads, lds);
   *wg_per_cu = n;
   return e;
 }

@@ -3,9 +3,9 @@
 // ppm_submit validates a request, copies its star, grid and energy grid,
 // charges its device memory against the context's budget and queues it.
 // A worker thread owned by the context takes, whenever the device is free,
-// every queued request that shares the oldest request's grid (up to
-// kMaxBatch) and evaluates them in one launch sequence (the batched
-// persistent work queue for two or more, the plain path for one), while
+// every queued request whose atmosphere is the oldest request's (up to
+// kMaxBatch, in arrival order), groups them by grid and evaluates all of
+// them in one persistent batched launch (several grids per launch), while
 // new requests keep arriving.  Results are copied to host memory of the
 // request and handed out by ppm_poll in any order.  The reference has no
 // serving layer; each request is one calc_one_star
@@ -15,6 +15,7 @@
 #include <memory>
 #include <unordered_map>
 
+#include "ppm_batch.hpp"
 #include "ppm_ctx.hpp"
 
 namespace ppm {
@@ -73,7 +74,8 @@ void worker_loop(ppm_ctx* c) {
   ServeState& S = *c->serve;
   std::vector<Request*> batch;
   std::vector<uint64_t> tickets;
-  std::vector<ppm_star> stars;
+  std::vector<std::vector<int>> members;  // per grid group: indices into batch
+  std::vector<std::vector<ppm_star>> stars;
   for (;;) {
     batch.clear();
     tickets.clear();
@@ -81,11 +83,11 @@ void worker_loop(ppm_ctx* c) {
       std::unique_lock<std::mutex> lk(S.mu);
       S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty(); });
       if (S.stop) return;
-      // the oldest request and every queued request on the same grid
-      Request* first = S.reqs.at(S.queue.front()).get();
+      // the oldest request and every queued request of its atmosphere
+      const int atm = S.reqs.at(S.queue.front())->star.atmosphere;
       for (auto it = S.queue.begin(); it != S.queue.end() && (int)batch.size() < kMaxBatch;) {
         Request* r = S.reqs.at(*it).get();
-        if (same_grid(*first, *r)) {
+        if (r->star.atmosphere == atm) {
           r->state = ReqState::kRunning;
           batch.push_back(r);
           tickets.push_back(*it);
@@ -96,60 +98,133 @@ void worker_loop(ppm_ctx* c) {
       }
     }
     const int n = (int)batch.size();
-    const Request& g0 = *batch[0];
-    ppm_grid grid = g0.grid;
-    grid.energies_keV = g0.energies.data();
-    grid.n_energy = (int)g0.energies.size();
-    grid.method = n > 1 ? PPM_METHOD_BATCHED : PPM_METHOD_PLAIN;
-    stars.resize(n);
-    for (int k = 0; k < n; ++k) stars[k] = batch[k]->star;
-    const size_t per_flux = (size_t)grid.n_phase * grid.n_energy, per_counts = (size_t)g0.n_ch * grid.n_phase;
-    ppm_status st = PPM_OK;
+    // group by grid, groups in order of their first request
+    members.clear();
+    for (int k = 0; k < n; ++k) {
+      size_t gi = 0;
+      while (gi < members.size() && !same_grid(*batch[members[gi][0]], *batch[k])) ++gi;
+      if (gi == members.size()) members.emplace_back();
+      members[gi].push_back(k);
+    }
+    const int n_groups = (int)members.size();
+    std::vector<size_t> f_off(n_groups + 1, 0), c_off(n_groups + 1, 0);
+    for (int gi = 0; gi < n_groups; ++gi) {
+      const Request& r0 = *batch[members[gi][0]];
+      const size_t m = members[gi].size();
+      f_off[gi + 1] = f_off[gi] + m * r0.grid.n_phase * r0.energies.size();
+      c_off[gi + 1] = c_off[gi] + m * (size_t)r0.n_ch * r0.grid.n_phase;
+    }
+    std::vector<ppm_status> st(n_groups, PPM_OK);
     {
       std::lock_guard<std::mutex> lk(c->exec_mu);
       hipSetDevice(c->device);
       int e = 0;
-      if ((e = S.d_flux.ensure(sizeof(double) * per_flux * n)) || (e = S.h_flux.ensure(sizeof(double) * per_flux * n)) ||
-          (per_counts && ((e = S.d_counts.ensure(sizeof(double) * per_counts * n)) ||
-                          (e = S.h_counts.ensure(sizeof(double) * per_counts * n))))) {
-        st = hip_fail(c, e, "serving buffers");
+      if ((e = S.d_flux.ensure(sizeof(double) * f_off[n_groups])) ||
+          (e = S.h_flux.ensure(sizeof(double) * f_off[n_groups])) ||
+          (c_off[n_groups] && ((e = S.d_counts.ensure(sizeof(double) * c_off[n_groups])) ||
+                               (e = S.h_counts.ensure(sizeof(double) * c_off[n_groups]))))) {
+        st.assign(n_groups, hip_fail(c, e, "serving buffers"));
       } else {
-        ppm_stats stats{};
-        st = eval_any(c, stars.data(), n, &grid, (double*)S.d_flux.p, per_counts ? (double*)S.d_counts.p : nullptr,
-                      &stats);
-        if (st == PPM_OK) {
-          if ((e = hipMemcpyAsync(S.h_flux.p, S.d_flux.p, sizeof(double) * per_flux * n, hipMemcpyDeviceToHost,
-                                  c->stream)) ||
-              (per_counts && (e = hipMemcpyAsync(S.h_counts.p, S.d_counts.p, sizeof(double) * per_counts * n,
-                                                 hipMemcpyDeviceToHost, c->stream))) ||
-              (e = hipStreamSynchronize(c->stream)))
-            st = hip_fail(c, e, "serving read-back");
+        std::vector<BatchGroup> groups(n_groups);
+        stars.assign(n_groups, {});
+        for (int gi = 0; gi < n_groups; ++gi) {
+          const Request& r0 = *batch[members[gi][0]];
+          for (int k : members[gi]) stars[gi].push_back(batch[k]->star);
+          BatchGroup& G = groups[gi];
+          G.stars = stars[gi].data();
+          G.n = (int)stars[gi].size();
+          G.g = r0.grid;
+          G.g.energies_keV = nullptr;
+          G.g.n_energy = (int)r0.energies.size();
+          G.energies = r0.energies;
+          phase_points(G.g, G.pts);
+          G.d_flux = (double*)S.d_flux.p + f_off[gi];
+          G.d_counts = r0.n_ch ? (double*)S.d_counts.p + c_off[gi] : nullptr;
+        }
+        // launches: one per class of grids (at most 128 energies or more,
+        // fine tables up to 384 nodes or more: the kernel instance and the
+        // LDS, hence the occupancy, follow the largest grid of a launch),
+        // groups packed in order while the LDS of one workgroup holds them;
+        // a group the batched kernel cannot take (energies not strictly
+        // increasing, or too large alone) runs on the plain path
+        std::vector<std::vector<int>> launches;
+        std::vector<int> plain;
+        for (int cls = 0; cls < 4; ++cls) {
+          std::vector<BatchGroup> cur;
+          std::vector<int> cur_idx;
+          for (int gi = 0; gi < n_groups; ++gi) {
+            const BatchGroup& G = groups[gi];
+            if (((G.energies.size() > 128) | (G.g.n_fine > 384) << 1) != cls) continue;
+            bool ascending = true;
+            for (size_t q = 1; q < G.energies.size(); ++q) ascending = ascending && G.energies[q] > G.energies[q - 1];
+            if (!ascending || !batch_groups_fit(c, {G})) {
+              plain.push_back(gi);
+              continue;
+            }
+            cur.push_back(G);
+            if (!batch_groups_fit(c, cur)) {
+              cur.pop_back();
+              launches.push_back(cur_idx);
+              cur.assign(1, G);
+              cur_idx.clear();
+            }
+            cur_idx.push_back(gi);
+          }
+          if (!cur_idx.empty()) launches.push_back(cur_idx);
+        }
+        for (const std::vector<int>& L : launches) {
+          std::vector<BatchGroup> lg;
+          for (int gi : L) lg.push_back(groups[gi]);
+          ppm_stats stats{};
+          const ppm_status q = eval_batched_groups(c, lg, &stats);
+          for (int gi : L) st[gi] = q;
+        }
+        for (int gi : plain) {
+          ppm_grid g = groups[gi].g;
+          g.energies_keV = groups[gi].energies.data();
+          g.method = PPM_METHOD_PLAIN;
+          ppm_stats stats{};
+          st[gi] = eval_any(c, groups[gi].stars, groups[gi].n, &g, groups[gi].d_flux, groups[gi].d_counts, &stats);
+        }
+        if ((e = hipMemcpyAsync(S.h_flux.p, S.d_flux.p, sizeof(double) * f_off[n_groups], hipMemcpyDeviceToHost,
+                                c->stream)) ||
+            (c_off[n_groups] && (e = hipMemcpyAsync(S.h_counts.p, S.d_counts.p, sizeof(double) * c_off[n_groups],
+                                                    hipMemcpyDeviceToHost, c->stream))) ||
+            (e = hipStreamSynchronize(c->stream))) {
+          const ppm_status q = hip_fail(c, e, "serving read-back");
+          for (ppm_status& x : st)
+            if (x == PPM_OK) x = q;
         }
       }
     }
     const Clock::time_point now = Clock::now();
     {
       std::lock_guard<std::mutex> lk(S.mu);
-      for (int k = 0; k < n; ++k) {
-        Request* r = batch[k];
-        S.charged -= r->bytes;
-        --S.active;
-        if (r->cancel_pending) {
-          S.reqs.erase(tickets[k]);
-          continue;
-        }
-        r->t_done = now;
-        r->status = st;
-        if (st == PPM_OK) {
-          const double* f = (const double*)S.h_flux.p + per_flux * k;
-          r->flux.assign(f, f + per_flux);
-          if (per_counts) {
-            const double* q = (const double*)S.h_counts.p + per_counts * k;
-            r->counts.assign(q, q + per_counts);
+      for (int gi = 0; gi < n_groups; ++gi) {
+        const size_t per_flux = (f_off[gi + 1] - f_off[gi]) / members[gi].size();
+        const size_t per_counts = (c_off[gi + 1] - c_off[gi]) / members[gi].size();
+        for (size_t j = 0; j < members[gi].size(); ++j) {
+          const int k = members[gi][j];
+          Request* r = batch[k];
+          S.charged -= r->bytes;
+          --S.active;
+          if (r->cancel_pending) {
+            S.reqs.erase(tickets[k]);
+            continue;
           }
-          r->state = ReqState::kDone;
-        } else {
-          r->state = ReqState::kFailed;
+          r->t_done = now;
+          r->status = st[gi];
+          if (st[gi] == PPM_OK) {
+            const double* f = (const double*)S.h_flux.p + f_off[gi] + per_flux * j;
+            r->flux.assign(f, f + per_flux);
+            if (per_counts) {
+              const double* q = (const double*)S.h_counts.p + c_off[gi] + per_counts * j;
+              r->counts.assign(q, q + per_counts);
+            }
+            r->state = ReqState::kDone;
+          } else {
+            r->state = ReqState::kFailed;
+          }
         }
       }
     }

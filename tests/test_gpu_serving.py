@@ -1,7 +1,8 @@
-"""Serving loop: requests on two grids submitted together are packed into
-launches per grid, complete out of order, and give the same profiles as a
-direct evaluation; cancel, drain and budget admission behave as declared in
-include/ppm.h."""
+"""Serving loop: queued requests on different grids are packed into one
+batched launch (several grids per launch; a grid the batched kernel cannot
+take runs on the plain path), complete out of order, and give the same
+profiles as a direct evaluation; cancel, drain and budget admission behave
+as declared in include/ppm.h."""
 import time
 
 import numpy as np
@@ -58,6 +59,34 @@ def test_mixed_grids_match_direct_evaluation(ctx, data_root, small_nsx_cfg):
         orc.close()
     # tickets are released by a successful poll
     assert ctx.poll(tickets[0], g1)[0] == abi.ERR_NOT_FOUND
+
+
+def test_many_grids_in_one_launch(ctx):
+    """A heavy request keeps the device busy while requests on five grids
+    queue; they are evaluated together (one launch for the four batchable
+    grids, the plain path for descending energies) and each equals its direct
+    evaluation."""
+    e50 = list(np.geomspace(0.3, 8.0, 50))
+    grids = [abi.make_grid(32, 16, sampling=abi.SAMPLE_GL, n_gl=2, response=1),
+             abi.make_grid(48, 8, sampling=abi.SAMPLE_GL, n_gl=2, response=0),
+             abi.make_grid(64, 4, n_fine=40, sampling=abi.SAMPLE_GL, n_gl=3, energies=e50, response=-1),
+             abi.make_grid(40, 12, sampling=abi.SAMPLE_POINTS, energies=e50[::-1], response=-1),
+             abi.make_grid(24, 32, sampling=abi.SAMPLE_GL, n_gl=1, response=1)]
+    heavy_g = abi.make_grid(256, 32, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
+    stars = S.sample_priors(16, seed=41)
+    print("case: serving 15 requests on 5 grids behind a heavy one", flush=True)
+    heavy = ctx.submit(stars[15], heavy_g)
+    todo = [(st, grids[k % len(grids)]) for k, st in enumerate(stars[:15])]
+    tickets = [ctx.submit(st, g) for st, g in todo]
+    ctx.drain()
+    assert wait(ctx, heavy, heavy_g)[0] == abi.OK
+    for k, ((st, g), t) in enumerate(zip(todo, tickets)):
+        status, flux, cnt, _ = wait(ctx, t, g)
+        assert status == abi.OK, k
+        ref_f, ref_c, _ = ctx.eval([st], g)
+        assert rel_err(flux, ref_f[0]) <= 1e-5, (k, rel_err(flux, ref_f[0]))
+        if g.response >= 0:
+            assert rel_err(cnt, ref_c[0]) <= 1e-5, k
 
 
 def test_cancel_and_unknown_ticket(ctx):
