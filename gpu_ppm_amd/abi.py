@@ -92,6 +92,10 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Fit(ctypes.Structure):
+    _fields_ = [("chi2", c_double), ("loglike", c_double), ("model_total", c_double)]
+
+
 class LaunchShape(ctypes.Structure):
     _fields_ = [("items_per_slot", c_int), ("wg_per_cu", c_int)]
 

@@ -360,7 +360,8 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
     const double scale = st.exposure_s * (1.0 / g.n_phase);  // cpu/devarshi24st_cpu.cpp:255
     if ((e = hipMemcpyAsync(d_scale_slot, &scale, sizeof(double), hipMemcpyHostToDevice, c->stream)))
       return hip_fail(c, e, "scale upload");
-    if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, 1, g.n_phase, d_scale_slot, d_counts, c->stream)))
+    if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, 1, g.n_phase, d_scale_slot, d_counts, c->stream,
+                         c->fit, c->fit_star)))
       return hip_fail(c, e, "fold launch");
   }
   if ((e = hipEventRecord(c->ev2, c->stream))) return hip_fail(c, e, "event");
@@ -422,6 +423,7 @@ ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   const size_t fstride = (size_t)grid->n_phase * n_e;
   const size_t cstride = grid->response >= 0 ? (size_t)c->resp[grid->response].n_ch * grid->n_phase : 0;
   for (int i = 0; i < n; ++i) {
+    c->fit_star = i;
     s = eval_plain(c, stars[i], *grid, energies, pts, d_flux + i * fstride,
                    d_counts ? d_counts + i * cstride : nullptr, (double*)c->ws.scale.p + i, stats);
     if (s) return s;
@@ -750,6 +752,50 @@ ppm_status ppm_fold_device(ppm_ctx* c, int response, int n_batch, int n_phase, c
                        d_counts, c->stream)))
     return hip_fail(c, e, "fold launch");
   if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
+  return PPM_OK;
+}
+
+ppm_status ppm_eval_fit(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, const double* data,
+                        int data_per_star, ppm_fit* fit, double* counts, ppm_stats* stats) {
+  if (!c || !stars || !grid || n < 1 || !data || !fit) return fail(c, PPM_ERR_ARG, "null argument");
+  if (grid->response < 0 || grid->response >= (int)c->resp.size())
+    return fail(c, PPM_ERR_ARG, "a fit needs the grid's response fold");
+  if (data_per_star != 0 && data_per_star != 1) return fail(c, PPM_ERR_ARG, "data_per_star must be 0 or 1");
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  hipSetDevice(c->device);
+  std::vector<double> energies;
+  ppm_status s = resolve_energies(c, *grid, energies);
+  if (s) return s;
+  const int n_ch = c->resp[grid->response].n_ch, blocks = fold_blocks(n_ch, grid->n_phase);
+  const size_t per = (size_t)n_ch * grid->n_phase;
+  const size_t nf = (size_t)n * grid->n_phase * energies.size(), nc = (size_t)n * per;
+  const size_t nd = (data_per_star ? (size_t)n : 1) * per;
+  Workspace& W = c->ws;
+  int e;
+  if ((e = W.out_flux.ensure(sizeof(double) * nf)) || (e = W.out_counts.ensure(sizeof(double) * nc)) ||
+      (e = W.fit_data.ensure(sizeof(double) * nd)) || (e = W.fit_part.ensure(sizeof(double) * 3 * n * blocks)) ||
+      (e = W.fit_out.ensure(sizeof(double) * 3 * n)))
+    return hip_fail(c, e, "device allocation");
+  if ((e = hipMemcpy(W.fit_data.p, data, sizeof(double) * nd, hipMemcpyHostToDevice)))
+    return hip_fail(c, e, "data upload");
+  const FoldFit F{(const double*)W.fit_data.p, data_per_star ? per : 0, (double*)W.fit_part.p, blocks, 0};
+  struct Reset {
+    ppm_ctx* c;
+    ~Reset() {
+      c->fit = nullptr;
+      c->fit_star = 0;
+    }
+  } reset{c};
+  c->fit = &F;
+  c->fit_star = 0;
+  if ((s = eval_any(c, stars, n, grid, (double*)W.out_flux.p, (double*)W.out_counts.p, stats))) return s;
+  std::vector<double> out(3 * (size_t)n);
+  if ((e = launch_fit_reduce((const double*)W.fit_part.p, blocks, n, (double*)W.fit_out.p, c->stream)) ||
+      (e = hipMemcpyAsync(out.data(), W.fit_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream)) ||
+      (counts && (e = hipMemcpyAsync(counts, W.out_counts.p, sizeof(double) * nc, hipMemcpyDeviceToHost, c->stream))) ||
+      (e = hipStreamSynchronize(c->stream)))
+    return hip_fail(c, e, "fit statistics");
+  for (int b = 0; b < n; ++b) fit[b] = ppm_fit{out[3 * b], out[3 * b + 1], out[3 * b + 2]};
   return PPM_OK;
 }
 

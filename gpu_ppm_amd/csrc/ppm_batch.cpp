@@ -519,6 +519,10 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   put(&B, sizeof B);
   put(&n_wg, sizeof n_wg);
   put(&lds, sizeof lds);
+  {
+    const FoldFit none{};
+    put(c->fit ? c->fit : &none, sizeof(FoldFit));
+  }
   for (int k = 0; k < n_groups; ++k) {
     put(&P[k], sizeof(PlainLaunch));
     put(&bg[k], sizeof(BatchGrid));
@@ -546,7 +550,8 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       if (!G.d_counts || G.g.response < 0) continue;
       const Response& R = c->resp[G.g.response];
       if ((r = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, G.d_flux, G.n, G.g.n_phase,
-                           (const double*)W.scale.p + group_star0[k], G.d_counts, c->stream)))
+                           (const double*)W.scale.p + group_star0[k], G.d_counts, c->stream, c->fit,
+                           group_star0[k])))
         return r;
     }
     if (timed && (r = hipEventRecord(c->ev2, c->stream))) return r;

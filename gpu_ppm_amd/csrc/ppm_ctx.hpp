@@ -36,9 +36,10 @@ struct HostBuf {
 
 struct Workspace {
   DevBuf spots, tasks, desc, ph, fine, pts, energies, partial, red, scale, out_flux, out_counts;
+  DevBuf fit_data, fit_part, fit_out;  // ppm_eval_fit
   void release_all() {
     for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &partial, &red, &scale, &out_flux,
-                      &out_counts})
+                      &out_counts, &fit_data, &fit_part, &fit_out})
       b->release();
   }
 };
@@ -105,6 +106,10 @@ struct ppm_ctx {
   ppm::ServeState* serve = nullptr;
   ppm::BatchState* batch_state = nullptr;
   bool no_graph = false;  // PPM_NO_GRAPH=1: launch the batched sequence directly
+  // set by ppm_eval_fit for the duration of one evaluation: the folds also
+  // accumulate fit statistics; fit_star is the first star of the next fold
+  const ppm::FoldFit* fit = nullptr;
+  int fit_star = 0;
 };
 
 namespace ppm {

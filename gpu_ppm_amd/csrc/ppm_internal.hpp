@@ -111,9 +111,20 @@ int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P
 int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream);
 int launch_reduce(const PlainLaunch& P, double* d_red, void* stream);
 int reduce_groups(int n_tasks);
+// Fit statistics against data computed in the fold's epilogue: per star
+// partial sums over the fold's blocks, reduced by launch_fit_reduce.
+struct FoldFit {
+  const double* data;  // [stars][n_ch][n_phase] (stride between stars; 0: one shared array)
+  size_t stride;
+  double* part;        // [stars][blocks][3]: sum m, sum (d-m)^2/m, sum d ln m - m - lnGamma(d+1)
+  int blocks;          // fold_blocks(n_ch, n_phase)
+  int pad;
+};
+int fold_blocks(int n_ch, int n_phase);
 int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_e,
                 const double* d_flux, int n_batch, int n_phase, const double* d_scale,
-                double* d_counts, void* stream);
+                double* d_counts, void* stream, const FoldFit* fit = nullptr, int star0 = 0);
+int launch_fit_reduce(const double* d_part, int blocks, int n, double* d_out, void* stream);
 int max_dynamic_lds(int device);
 size_t integral_lds_bytes(int n_fine, int atmosphere, int nsx_rows);
 int energies_per_chunk();

@@ -679,7 +679,7 @@ __global__ __launch_bounds__(64) void k_fold(const uint16_t* __restrict__ rsp_hi
                                              const uint16_t* __restrict__ rsp_lo, int n_ch,
                                              int n_e, const double* __restrict__ flux, int n_phase,
                                              const double* __restrict__ scale,
-                                             double* __restrict__ counts) {
+                                             double* __restrict__ counts, FoldFit fit, int star0) {
   const int p0 = blockIdx.x * 16, b = blockIdx.y, t0 = blockIdx.z * kFoldTiles;
   const int l = threadIdx.x, row = l & 15, kq = (l >> 4) * 4;
   const int ph = p0 + row;
@@ -715,14 +715,61 @@ __global__ __launch_bounds__(64) void k_fold(const uint16_t* __restrict__ rsp_hi
     }
   }
   const double sc = scale[b];
+  // epilogue: the model counts, and with data the partial sums of the fit
+  // statistics over this wave's tile (FP64, wave reduction in fixed order)
+  const double* dat = fit.data ? fit.data + (size_t)(star0 + b) * fit.stride : nullptr;
+  double s_m = 0.0, s_chi = 0.0, s_ll = 0.0;
 #pragma unroll
   for (int t = 0; t < kFoldTiles; ++t)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int c = (t0 + t) * 16 + kq + i, p = p0 + row;
-      if (t < n_tiles && c < n_ch && p < n_phase)
-        counts[((size_t)b * n_ch + c) * n_phase + p] = sc * (double)acc[t][i];
+      if (t < n_tiles && c < n_ch && p < n_phase) {
+        const double m = sc * (double)acc[t][i];
+        if (counts) counts[((size_t)b * n_ch + c) * n_phase + p] = m;
+        if (dat) {
+          const double d = dat[(size_t)c * n_phase + p], r = d - m;
+          s_m += m;
+          if (m != 0.0 || d != 0.0) {  // an empty bin of an empty model adds 0 (the limit m -> 0)
+            s_chi += r * r / m;
+            s_ll += (d == 0.0 ? 0.0 : d * log(m)) - m - lgamma(d + 1.0);
+          }
+        }
+      }
     }
+  if (dat) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      s_m += __shfl_xor(s_m, o);
+      s_chi += __shfl_xor(s_chi, o);
+      s_ll += __shfl_xor(s_ll, o);
+    }
+    if (l == 0) {
+      double* out = fit.part + ((size_t)(star0 + b) * fit.blocks + blockIdx.z * gridDim.x + blockIdx.x) * 3;
+      out[0] = s_m;
+      out[1] = s_chi;
+      out[2] = s_ll;
+    }
+  }
+}
+
+// Fit statistics per star from the fold's partial sums (fixed order):
+// chi2 of cal_chi2 (p_devarshi24st.py:43-49: model and data scaled so the
+// model sums to 1e6, i.e. 1e6 / sum(m) * sum (d - m)^2 / m), the Poisson
+// log-likelihood and the model total.
+__global__ void k_fit_reduce(const double* __restrict__ part, int blocks, int n, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double s_m = 0.0, s_chi = 0.0, s_ll = 0.0;
+  for (int k = 0; k < blocks; ++k) {
+    const double* q = part + ((size_t)b * blocks + k) * 3;
+    s_m += q[0];
+    s_chi += q[1];
+    s_ll += q[2];
+  }
+  out[3 * b + 0] = 1e6 / s_m * s_chi;
+  out[3 * b + 1] = s_ll;
+  out[3 * b + 2] = s_m;
 }
 
 }  // namespace
@@ -777,13 +824,24 @@ int launch_reduce(const PlainLaunch& P, double* d_red, void* stream) {
   return hipGetLastError();
 }
 
+int fold_blocks(int n_ch, int n_phase) {
+  const int n_tiles = (n_ch + 15) / 16;
+  return ((n_phase + 15) / 16) * ((n_tiles + kFoldTiles - 1) / kFoldTiles);
+}
+
 int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_e,
                 const double* d_flux, int n_batch, int n_phase, const double* d_scale,
-                double* d_counts, void* stream) {
+                double* d_counts, void* stream, const FoldFit* fit, int star0) {
   const int n_tiles = (n_ch + 15) / 16;
   dim3 grid((n_phase + 15) / 16, n_batch, (n_tiles + kFoldTiles - 1) / kFoldTiles);
+  const FoldFit f = fit ? *fit : FoldFit{};
   hipLaunchKernelGGL(k_fold, grid, dim3(64), 0, (hipStream_t)stream, rsp_hi, rsp_lo, n_ch, n_e,
-                     d_flux, n_phase, d_scale, d_counts);
+                     d_flux, n_phase, d_scale, d_counts, f, star0);
+  return hipGetLastError();
+}
+
+int launch_fit_reduce(const double* d_part, int blocks, int n, double* d_out, void* stream) {
+  hipLaunchKernelGGL(k_fit_reduce, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, d_part, blocks, n, d_out);
   return hipGetLastError();
 }
 

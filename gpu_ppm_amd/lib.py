@@ -68,6 +68,9 @@ def load():
     L.ppm_cancel.argtypes = [c_void_p, c_uint64]
     L.ppm_drain.restype = c_int
     L.ppm_drain.argtypes = [c_void_p]
+    L.ppm_eval_fit.restype = c_int
+    L.ppm_eval_fit.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), POINTER(c_double), c_int,
+                               POINTER(abi.Fit), POINTER(c_double), POINTER(abi.Stats)]
     L.ppm_tune.restype = c_int
     L.ppm_tune.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), c_int, POINTER(abi.TuneResult)]
     for f in (L.ppm_set_launch_shape, L.ppm_get_launch_shape):
@@ -171,6 +174,24 @@ class Context:
         self._check(load().ppm_eval(self._h, arr, n, byref(grid), _dptr(flux),
                                     _dptr(cnt) if cnt is not None else None, byref(stats)))
         return flux, cnt, stats
+
+    def eval_fit(self, stars, grid, data, counts=False):
+        """Fit statistics of the stars' counts against data ([n_ch][n_phase]
+        shared, or [n][n_ch][n_phase]); returns (chi2, loglike, model_total)
+        arrays of length n, the counts if asked for, and the stats."""
+        arr = _star_array(stars)
+        n = len(arr)
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        per_star = 1 if d.ndim == 3 else 0
+        fits = (abi.Fit * n)()
+        cnt = None
+        if counts:
+            cnt = np.zeros((n, self.response_info(grid.response)[0], grid.n_phase))
+        stats = abi.Stats()
+        self._check(load().ppm_eval_fit(self._h, arr, n, byref(grid), _dptr(d), per_star, fits,
+                                        _dptr(cnt) if cnt is not None else None, byref(stats)))
+        out = np.array([[f.chi2, f.loglike, f.model_total] for f in fits])
+        return out[:, 0], out[:, 1], out[:, 2], cnt, stats
 
     def eval_device(self, stars, grid, d_flux, d_counts=None):
         """Outputs to device pointers (ints, e.g. gpu_ppm_amd.hip.DeviceArray.ptr)."""

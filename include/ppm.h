@@ -203,6 +203,29 @@ ppm_status ppm_cancel(ppm_ctx* ctx, uint64_t ticket);
 /* Block until every admitted request is finished or cancelled. */
 ppm_status ppm_drain(ppm_ctx* ctx);
 
+/* Fit of one star's folded counts m[c][p] to data d[c][p]. */
+typedef struct ppm_fit {
+  double chi2;        /* sum (d - m)^2 / m with model and data scaled so that the model sums
+                         to 1e6: cal_chi2 of p_devarshi24st.py:43-49                      */
+  double loglike;     /* Poisson log-likelihood: sum d ln m - m - ln Gamma(d + 1)         */
+  double model_total; /* sum of m                                                         */
+} ppm_fit;
+
+/* Evaluate n profiles and fit their counts to data, the statistics summed in
+ * the response fold's epilogue on the device (FP64, fixed order).  data is
+ * [n_ch][n_phase] for every star (data_per_star = 0) or [n][n_ch][n_phase]
+ * (1); fit receives n entries; counts [n][n_ch][n_phase] if non-NULL.  The
+ * grid needs a response.  The statistics follow the formulas literally (as
+ * cal_chi2 does), except that a bin with m = 0 and d = 0 adds 0 to both (the
+ * limit m -> 0; cal_chi2 would give 0/0): a bin with m < 0, which the cubic
+ * energy interpolant can produce in a steep Wien tail, adds a negative chi2
+ * term and makes the log-likelihood NaN; m = 0 < d gives infinities.  No
+ * counterpart in the reference,
+ * which computes cal_chi2 in its plotting script (p_devarshi24st.py:43-49)
+ * from the written counts files. */
+ppm_status ppm_eval_fit(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid, const double* data,
+                        int data_per_star, ppm_fit* fit, double* counts, ppm_stats* stats);
+
 /* Launch shape of the batched path.  A 0 field means the default: 12
  * work-queue items per resident workgroup (each spot's rings are cut into
  * chunks of similar cost so that the queue holds about that many items per

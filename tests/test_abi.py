@@ -141,3 +141,13 @@ def test_tuner_entry_points_reject_bad_arguments():
     assert L.ppm_tune(None, ctypes.byref(st), 1, ctypes.byref(g), 0, ctypes.byref(r)) == abi.ERR_ARG
     assert L.ppm_set_launch_shape(None, ctypes.byref(g), 1, abi.ATM_NSX, ctypes.byref(sh)) == abi.ERR_ARG
     assert L.ppm_get_launch_shape(None, ctypes.byref(g), 1, abi.ATM_NSX, ctypes.byref(sh)) == abi.ERR_ARG
+
+
+def test_fit_entry_point_rejects_bad_arguments():
+    L = lib.load()
+    g = abi.make_grid(16, 8, response=0)
+    st = abi.make_star([abi.make_spot(0.5, 0.2, 0.3)])
+    fit = abi.Fit()
+    data = (ctypes.c_double * 4)()
+    assert L.ppm_eval_fit(None, ctypes.byref(st), 1, ctypes.byref(g), data, 0, ctypes.byref(fit), None,
+                          None) == abi.ERR_ARG
