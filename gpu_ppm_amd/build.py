@@ -64,6 +64,8 @@ def build_lib():
 def build_apps():
     os.makedirs(BIN, exist_ok=True)
     apps = os.path.join(PKG, "apps")
+    # host-only tool (no device code): the response builder
+    _run(["g++"] + HOST_FLAGS + ["-O2", "-o", os.path.join(BIN, "ppm_response"), os.path.join(apps, "ppm_response.cpp")])
     for name in ("os1_gpu", "devarshi24st_gpu", "ppm_bench_host"):
         src = os.path.join(apps, name + ".cpp")
         if not os.path.exists(src):
