@@ -22,6 +22,11 @@ ERR_UNSUPPORTED = 10
 
 ATM_BLACKBODY = 0
 ATM_NSX = 1
+NSX_2L2C_LE = 0  # NSX interpolants (ppm_star.nsx_interp)
+NSX_4C = 1
+NSX_4C_SG = 2
+NSX_4C_LE = 3
+NSX_4L = 4
 BEAM_ISOTROPIC = ord("i")
 BEAM_COS2 = ord("c")
 BEAM_SIN2 = ord("s")
@@ -54,7 +59,7 @@ class Star(ctypes.Structure):
         ("mass_msun", c_double), ("radius_eq_km", c_double), ("spin_hz", c_double),
         ("inclination_rad", c_double), ("distance_km", c_double), ("exposure_s", c_double),
         ("logg_override", c_double), ("point_source_below", c_double),
-        ("atmosphere", c_int), ("variant", c_int), ("n_spots", c_int), ("reserved", c_int),
+        ("atmosphere", c_int), ("variant", c_int), ("n_spots", c_int), ("nsx_interp", c_int),
         ("spot", Spot * PPM_MAX_SPOTS),
     ]
 
@@ -107,6 +112,9 @@ class TuneResult(ctypes.Structure):
 
 NSX_H_V200804 = dict(n_logT=35, n_logg=14, n_logEkT=166, n_mu=67, logT_min=5.10, logT_max=6.80,
                      logg_min=13.7, logg_max=15.0, logEkT_min=-1.30, logEkT_max=2.00)
+# cpu/nsx.hpp:39-51
+NSX_H_V171019 = dict(n_logT=35, n_logg=11, n_logEkT=166, n_mu=67, logT_min=5.10, logT_max=6.80,
+                     logg_min=13.7, logg_max=14.7, logEkT_min=-1.30, logEkT_max=2.00)
 
 
 def nsx_config(**kw):
@@ -123,11 +131,11 @@ def make_spot(theta1, angrad1, kT_keV, phi1=0.0, theta2=0.0, phi2=0.0, angrad2=0
 
 def make_star(spots, mass_msun=1.4, radius_eq_km=12.0, spin_hz=600.0, inclination_rad=1.0471975511965976,
               distance_km=KPC_KM, exposure_s=1e6, logg_override=0.0, point_source_below=0.0,
-              atmosphere=ATM_BLACKBODY, variant=VARIANT_OS1):
+              atmosphere=ATM_BLACKBODY, variant=VARIANT_OS1, nsx_interp=NSX_2L2C_LE):
     st = Star(mass_msun=mass_msun, radius_eq_km=radius_eq_km, spin_hz=spin_hz,
               inclination_rad=inclination_rad, distance_km=distance_km, exposure_s=exposure_s,
               logg_override=logg_override, point_source_below=point_source_below,
-              atmosphere=atmosphere, variant=variant, n_spots=len(spots), reserved=0)
+              atmosphere=atmosphere, variant=variant, n_spots=len(spots), nsx_interp=nsx_interp)
     if not 1 <= len(spots) <= PPM_MAX_SPOTS:
         raise ValueError("1..%d spots" % PPM_MAX_SPOTS)
     for k, sp in enumerate(spots):

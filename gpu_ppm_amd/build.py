@@ -66,7 +66,7 @@ def build_apps():
     apps = os.path.join(PKG, "apps")
     # host-only tool (no device code): the response builder
     _run(["g++"] + HOST_FLAGS + ["-O2", "-o", os.path.join(BIN, "ppm_response"), os.path.join(apps, "ppm_response.cpp")])
-    for name in ("os1_gpu", "devarshi24st_gpu", "ppm_bench_host"):
+    for name in ("os1_gpu", "devarshi24st_gpu", "t_interp_gpu"):
         src = os.path.join(apps, name + ".cpp")
         if not os.path.exists(src):
             continue

@@ -92,6 +92,10 @@ ppm_status validate(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, int n_e_r
     return fail(c, PPM_ERR_ARG, "unknown atmosphere");
   if (st.atmosphere == PPM_ATM_NSX && !c->N.data)
     return fail(c, PPM_ERR_UNSUPPORTED, "NSX atmosphere requested but no NSX grid is loaded");
+  if (st.atmosphere == PPM_ATM_NSX && (st.nsx_interp < PPM_NSX_2L2C_LE || st.nsx_interp > PPM_NSX_4L))
+    return fail(c, PPM_ERR_ARG, "unknown NSX interpolant");
+  if (st.atmosphere == PPM_ATM_NSX && nsx_cubic_tg(st.nsx_interp) && (c->N.nT < 4 || c->N.ng < 4))
+    return fail(c, PPM_ERR_ARG, "cubic logT / log g stencils need 4 nodes on each axis");
   if (st.variant != PPM_VARIANT_OS1 && st.variant != PPM_VARIANT_DEVARSHI)
     return fail(c, PPM_ERR_ARG, "unknown variant");
   // the surface must lie outside the Schwarzschild radius and rotate below c
@@ -165,6 +169,7 @@ void spot_params(const ppm_ctx* c, const ppm_star& st, const ppm_spot& sp, int n
     P.kT_nsx = P.kT;
   }
   P.n_side = n_side;
+  P.nsx_interp = st.atmosphere == PPM_ATM_NSX ? st.nsx_interp : PPM_NSX_2L2C_LE;
 }
 
 // Candidate rings of a spot: rings whose colatitude is within angrad1 of the
@@ -341,6 +346,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   P.n_e = n_e;
   P.n_echunks = (n_e + energies_per_chunk() - 1) / energies_per_chunk();
   P.atmosphere = st.atmosphere;
+  P.emode = st.atmosphere == PPM_ATM_NSX ? nsx_energy_mode(st.nsx_interp) : kECubic;
   P.nsx_rows = nsx_rows;
 
   if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");

@@ -156,16 +156,39 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
 ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats) {
-  std::vector<BatchGroup> groups(1);
-  BatchGroup& G = groups[0];
-  G.stars = stars;
-  G.n = n;
-  G.g = g;
-  G.energies = energies;
-  G.pts = pts;
-  G.d_flux = d_flux;
-  G.d_counts = d_counts;
-  return eval_groups_rows(c, groups, stats, 0);
+  // one launch per launch class present, each over the runs of consecutive
+  // stars of its class
+  std::vector<int> classes;
+  for (int i = 0; i < n; ++i)
+    if (std::find(classes.begin(), classes.end(), batch_class(stars[i])) == classes.end())
+      classes.push_back(batch_class(stars[i]));
+  const size_t fstride = (size_t)g.n_phase * energies.size();
+  const size_t cstride = d_counts && g.response >= 0 ? (size_t)c->resp[g.response].n_ch * g.n_phase : 0;
+  for (const int cl : classes) {
+    std::vector<BatchGroup> groups;
+    for (int i = 0; i < n;) {
+      if (batch_class(stars[i]) != cl) {
+        ++i;
+        continue;
+      }
+      int j = i;
+      while (j < n && batch_class(stars[j]) == cl) ++j;
+      BatchGroup G;
+      G.stars = stars + i;
+      G.n = j - i;
+      G.g = g;
+      G.energies = energies;
+      G.pts = pts;
+      G.d_flux = d_flux + i * fstride;
+      G.d_counts = d_counts ? d_counts + i * cstride : nullptr;
+      G.fit_star0 = i;
+      groups.push_back(std::move(G));
+      i = j;
+    }
+    const ppm_status s = eval_groups_rows(c, groups, stats, 0);
+    if (s) return s;
+  }
+  return PPM_OK;
 }
 
 ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats) {
@@ -266,12 +289,14 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   for (const BatchGroup& G : groups) n += G.n;
   if (!n_groups || n < 1) return fail(c, PPM_ERR_ARG, "no stars");
   const int atm = groups[0].stars[0].atmosphere;
+  const int emode = batch_emode(groups[0].stars[0]);
   for (const BatchGroup& G : groups) {
     for (size_t e = 1; e < G.energies.size(); ++e)
       if (!(G.energies[e] > G.energies[e - 1]))
         return fail(c, PPM_ERR_ARG, "the batched path needs strictly increasing energies");
     for (int i = 0; i < G.n; ++i)
-      if (G.stars[i].atmosphere != atm) return fail(c, PPM_ERR_ARG, "a batch needs one atmosphere model");
+      if (G.stars[i].atmosphere != atm || batch_emode(G.stars[i]) != emode)
+        return fail(c, PPM_ERR_ARG, "a launch needs one atmosphere and one energy-axis interpolant");
   }
   int e;
   if (!BS->n_cu && hipDeviceGetAttribute(&BS->n_cu, hipDeviceAttributeMultiprocessorCount, c->device))
@@ -304,7 +329,6 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       for (int q = 0; q < G.stars[s].n_spots; ++q) {
         SpotParams P;
         spot_params(c, G.stars[s], G.stars[s].spot[q], g.n_side, P);
-        P.pad = sg;
         const int si = (int)sp.size();
         sp.push_back(P);
         spot_first_task.push_back((int)tasks.size());
@@ -340,9 +364,9 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   int nsx_rows, nf_max, cov_words;
   const size_t lds = launch_geometry(c, groups, atm, rows_override, nsx_rows, nf_max, cov_words);
   if (lds > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
-  if ((e = batched_prepare(atm, epl, lds))) return hip_fail(c, e, "kernel attributes");
+  if ((e = batched_prepare(atm, epl, emode, lds))) return hip_fail(c, e, "kernel attributes");
   int wg_per_cu = 0;
-  if (batched_occupancy(atm, epl, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
+  if (batched_occupancy(atm, epl, emode, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
   BS->occ_wg_max = wg_per_cu;
   ppm_launch_shape shape{0, 0};
   if (BS->forced) {
@@ -508,6 +532,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   B.ev_cap = kEvCap;
   B.epl = epl;
   B.atmosphere = atm;
+  B.emode = emode;
   const int n_wg = std::max(1, std::min(slots, n_items));
 
   // ---- the launch sequence, captured once per configuration and replayed
@@ -530,6 +555,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
     put(&groups[k].d_counts, sizeof(void*));
     put(&groups[k].g.response, sizeof(int));
     put(&group_star0[k], sizeof(int));
+    put(&groups[k].fit_star0, sizeof(int));
   }
   // timed: per-kernel events around a direct launch; graph replays are timed
   // as a whole by events recorded around hipGraphLaunch (outside the graph)
@@ -551,7 +577,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       const Response& R = c->resp[G.g.response];
       if ((r = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, G.d_flux, G.n, G.g.n_phase,
                            (const double*)W.scale.p + group_star0[k], G.d_counts, c->stream, c->fit,
-                           group_star0[k])))
+                           G.fit_star0 >= 0 ? G.fit_star0 : group_star0[k])))
         return r;
     }
     if (timed && (r = hipEventRecord(c->ev2, c->stream))) return r;

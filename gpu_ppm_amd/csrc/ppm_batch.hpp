@@ -7,6 +7,14 @@
 
 namespace ppm {
 
+// Stars of one batched launch share a launch class: the atmosphere and, for
+// NSX, the energy-axis evaluation of the interpolant (a kernel instance); the
+// mu and (logT, log g) stencils are per spot.
+inline int batch_emode(const ppm_star& s) {
+  return s.atmosphere == PPM_ATM_NSX ? nsx_energy_mode(s.nsx_interp) : kECubic;
+}
+inline int batch_class(const ppm_star& s) { return s.atmosphere == PPM_ATM_NSX ? 1 + batch_emode(s) : 0; }
+
 ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats);
@@ -20,9 +28,10 @@ struct BatchGroup {
   std::vector<PhasePoint> pts;
   double* d_flux = nullptr;       // [n][n_phase][n_e]
   double* d_counts = nullptr;     // [n][n_ch][n_phase], or null (no fold)
+  int fit_star0 = -1;             // fit statistics: index of stars[0] in ppm_ctx::fit (-1: its launch index)
 };
 
-// One persistent launch over the stars of several grids (one atmosphere,
+// One persistent launch over the stars of several grids (one launch class,
 // energies strictly increasing in every group).
 ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats);
 

@@ -63,6 +63,7 @@ struct BatchLaunch {
   int n_fine_max;          // LDS layout: the largest grid of the launch
   int nsx_rows, cov_words, ev_cap, atmosphere;
   int epl;                 // energies per lane: 6, or 2 when every grid has <= 128 energies
+  int emode;               // NsxEnergyMode shared by the launch's stars
 };
 
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
@@ -70,10 +71,10 @@ int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 int batch_energies(int epl);  // energies per item
-int batched_prepare(int atmosphere, int epl, size_t lds);  // outside any stream capture
+int batched_prepare(int atmosphere, int epl, int emode, size_t lds);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);
 // flux[s - s0][b][e] for the stars [s0, s1) of one grid (G, its host copy).
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream);
-int batched_occupancy(int atmosphere, int epl, size_t lds, int* wg_per_cu);
+int batched_occupancy(int atmosphere, int epl, int emode, size_t lds, int* wg_per_cu);
 
 }  // namespace ppm

@@ -125,7 +125,7 @@ __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
   }
 }
 
-template <int ATM, int EPL>
+template <int ATM, int EPL, int EM>
 __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, BatchLaunch B) {
   constexpr int kBEPL = EPL;
   constexpr int kBEnergies = 64 * EPL;
@@ -217,17 +217,14 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           if (tid == 0) atomicOr(&B.desc[task].flags, 2);
           n_rows = rows_alloc;
         }
-        ax.t_lo = (float)(-r_lo);
-        ax.t_hi = (float)(N.nE - 1 - r_lo);
-        ax.st_lo = 0;
-        ax.st_hi = min(N.nE - 4 - r_lo, n_rows - 4);
-        ax.i0_max = rows_alloc - 63;  // rows_alloc >= 64
-        stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBThreads);
+        ax = energy_axis<EM>(N.nE, r_lo, n_rows, rows_alloc);
+        stage_nsx_rows(N, *Dp, r_lo, n_rows, s_S, tid, kBThreads);
       }
       __syncthreads();
       const TxMap txm{(float)((l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo),
                       (float)(inv_h * 0.30102999566398120)};
       const float inv_kT = (float)S.inv_kT;
+      const int mu_mode = nsx_mu_mode(S.nsx_interp);
       const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
       const bool point = T.ring == 0;
       const int n = point ? 1 : Dp->n_pix;
@@ -373,7 +370,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             const int k = lane < cnt ? s_q[lane] : 0;
             PatchVals pv;
             pv.im = -1;
-            if (lane < cnt) patch_setup<ATM>(phase_of(k), 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
+            if (lane < cnt) patch_setup<ATM>(phase_of(k), 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, mu_mode, txm, pv);
             // skipped patches add nothing to the prefix: the events between
             // two queued positions fire with the same prefix
             uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
@@ -404,15 +401,15 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 if (rows_path) {
                   if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next[0]);
                   if ((interior >> q[0]) & 1)
-                    nsx_eval_loaded<kBEPL, 1, false, kBSets>(le, pv, pr, q, ax, p_cur, lane, val);
+                    nsx_eval_loaded<kBEPL, 1, false, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, val);
                   else
-                    nsx_eval_loaded<kBEPL, 1, true, kBSets>(le, pv, pr, q, ax, p_cur, lane, val);
+                    nsx_eval_loaded<kBEPL, 1, true, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, val);
                 } else {
                   float w[4];
 #pragma unroll
                   for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[0]);
-                  nsx_eval_edge<kBEPL>(le, asc, readlane_f(pv.tx, q[0]), readlane_i(pv.im, q[0]), w, ax, s_S, s_C,
-                                       lane, val[0]);
+                  nsx_eval_edge<kBEPL, EM>(le, asc, readlane_f(pv.tx, q[0]), readlane_i(pv.im, q[0]), w, ax, s_S,
+                                           s_C, lane, val[0]);
                 }
               } else {
                 const float c = readlane_f(pv.w[0], q[0]), z = readlane_f(pv.zf, q[0]);
@@ -535,35 +532,36 @@ size_t batch_seg_doubles() { return (size_t)kBWP * kMaxEPL * 64; }
 int batch_energies(int epl) { return 64 * epl; }
 
 namespace {
-const void* batched_fn(int atmosphere, int epl) {
-  if (atmosphere == PPM_ATM_NSX)
-    return epl == 2 ? (const void*)k_integral_batched<PPM_ATM_NSX, 2> : (const void*)k_integral_batched<PPM_ATM_NSX, 6>;
-  return epl == 2 ? (const void*)k_integral_batched<PPM_ATM_BLACKBODY, 2>
-                  : (const void*)k_integral_batched<PPM_ATM_BLACKBODY, 6>;
+template <int ATM, int EM>
+const void* batched_fn_epl(int epl) {
+  return epl == 2 ? (const void*)k_integral_batched<ATM, 2, EM> : (const void*)k_integral_batched<ATM, 6, EM>;
+}
+// The kernel instance of a launch: atmosphere, energies per lane and the
+// energy-axis mode of the NSX interpolant.
+const void* batched_fn(int atmosphere, int epl, int emode) {
+  if (atmosphere != PPM_ATM_NSX) return batched_fn_epl<PPM_ATM_BLACKBODY, kECubic>(epl);
+  switch (emode) {
+    case kECubicClip: return batched_fn_epl<PPM_ATM_NSX, kECubicClip>(epl);
+    case kELinear: return batched_fn_epl<PPM_ATM_NSX, kELinear>(epl);
+    default: return batched_fn_epl<PPM_ATM_NSX, kECubic>(epl);
+  }
 }
 }  // namespace
 
-int batched_prepare(int atmosphere, int epl, size_t lds) {
-  return hipFuncSetAttribute(batched_fn(atmosphere, epl), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+int batched_prepare(int atmosphere, int epl, int emode, size_t lds) {
+  return hipFuncSetAttribute(batched_fn(atmosphere, epl, emode), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, B.queue, B.n_evals);
   const size_t lds = batch_lds_bytes(B.n_fine_max, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap);
-  const bool nsx = B.atmosphere == PPM_ATM_NSX;
-  if (B.epl == 2) {
-    if (nsx)
-      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_NSX, 2>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
-    else
-      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_BLACKBODY, 2>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
-  } else {
-    if (nsx)
-      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_NSX, 6>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
-    else
-      hipLaunchKernelGGL((k_integral_batched<PPM_ATM_BLACKBODY, 6>), dim3(n_wg), dim3(kBThreads), lds, st, N, B);
-  }
-  return hipGetLastError();
+  DevNsx n_arg = N;
+  BatchLaunch b_arg = B;
+  void* args[] = {&n_arg, &b_arg};
+  const hipError_t e =
+      hipLaunchKernel(batched_fn(B.atmosphere, B.epl, B.emode), dim3(n_wg), dim3(kBThreads), args, lds, st);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream) {
@@ -574,9 +572,10 @@ int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int 
   return hipGetLastError();
 }
 
-int batched_occupancy(int atmosphere, int epl, size_t lds, int* wg_per_cu) {
+int batched_occupancy(int atmosphere, int epl, int emode, size_t lds, int* wg_per_cu) {
   int n = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl), kBThreads, lds);
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl, emode), kBThreads, lds);
   *wg_per_cu = n;
   return e;
 }

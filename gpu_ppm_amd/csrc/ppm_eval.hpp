@@ -43,8 +43,11 @@ struct RingTab {
 // Set-up of one patch at rotational phase pp (already wrapped into
 // [ph[0], ph[0] + 1)): interpolation of the fine table
 // (cpu/devarshi24st_cpu.cpp:224-230), the mu stencil with its weights
-// (cpu/nsx.hpp:248-282) pre-multiplied by flux/I x weight, and the patch's
-// offset on the local energy axis.  im < 0 marks a skipped patch
+// pre-multiplied by flux/I x weight, and the patch's offset on the local
+// energy axis.  The mu stencil (NsxMuMode) is cubic Lagrange from one node
+// below the bracket, clamped to [0, nm-4] (cpu/nsx.hpp:181, 238-242), linear
+// in the bracket (cpu/nsx.hpp:305-313), or cubic with the linear bracket in
+// the last five (cpu/nsx.hpp:226-243, 260-271).  im < 0 marks a skipped patch
 // (flux/I == 0, cpu/devarshi24st_cpu.cpp:226).
 struct PatchVals {
   float tx;       // local energy-axis offset of this patch
@@ -69,7 +72,8 @@ __device__ inline double patch_flux_factor(double pp, const RingTab& R, int& i, 
 
 template <int ATM>
 __device__ inline void patch_setup(double pp, double weight, const RingTab& R, const double* s_mu,
-                                   const double* mu_den, int nm, double mu_min, TxMap txm, PatchVals& o) {
+                                   const double* mu_den, int nm, double mu_min, int mu_mode, TxMap txm,
+                                   PatchVals& o) {
   o.tx = 0.f;
   o.w[0] = o.w[1] = o.w[2] = o.w[3] = 0.f;
   o.zf = 0.f;
@@ -86,10 +90,10 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
     if (mu < mu_min) mu = mu_min;
     double am;
     int im = vec_hunt(s_mu, nm, mu, am);
-    if (im >= nm - 5) {  // linear at the edge
+    if (mu_mode == kMuLinear || (mu_mode == kMuCubicLinearEdge && im >= nm - 5)) {  // linear
       o.w[0] = (float)(c * (1 - am));
       o.w[1] = (float)(c * am);
-    } else {  // cubic in the middle
+    } else {  // cubic
       im = clampi(im - 1, 0, nm - 4);
       const double d0 = mu - s_mu[im], d1 = mu - s_mu[im + 1], d2 = mu - s_mu[im + 2], d3 = mu - s_mu[im + 3];
       const double* den = mu_den + 4 * im;
@@ -108,20 +112,50 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
 }
 
 // Energy-axis geometry of one workgroup: the staged rows of the (logT, log g)
-// blended NSX slice and the clamps of the reference's hunt (the stencil start
-// clamps to [0, nE-4] and logEkT to the axis ends, cpu/nsx.hpp:252-256).
+// blended NSX slice and the clamps of the reference's hunt (logEkT clamps to
+// the axis ends, cpu/hunt.hpp:160-172).  An energy at local axis position t
+// (clamped to [t_lo, t_hi]) uses row f = floor(t) clamped to [f_lo, f_hi] and
+// x = t - f: the cubic stencil rows f-1..f+2 (its start clamped to [0, nE-4],
+// cpu/nsx.hpp:180, 252-256) or the linear bracket f, f+1 (cpu/nsx.hpp:304).
 struct EAxis {
   float t_lo, t_hi;  // local t clamped to the axis ends
-  int st_lo, st_hi;  // legal local stencil starts (also the staged range)
+  int f_lo, f_hi;    // legal f (within the axis and the staged rows)
   int i0_max;        // largest row-set start whose 64 rows lie in the staged allocation
 };
 
-// Intensity at local axis position t (cubic Lagrange over 4 rows x 4 mu
-// columns; weights already carry flux/I).  `rows` points at the staged slice
-// row 0 plus the patch's first mu column.
+// Axis of rows [r_lo, r_lo + n_rows) of an nE-row grid, staged in rows_alloc.
+template <int EM>
+__device__ inline EAxis energy_axis(int nE, int r_lo, int n_rows, int rows_alloc) {
+  EAxis ax;
+  ax.t_lo = (float)(-r_lo);
+  ax.t_hi = (float)(nE - 1 - r_lo);
+  if (EM == kELinear) {
+    ax.f_lo = 0;
+    ax.f_hi = min(nE - 2 - r_lo, n_rows - 2);
+  } else {
+    ax.f_lo = 1;
+    ax.f_hi = min(nE - 4 - r_lo, n_rows - 4) + 1;
+  }
+  ax.i0_max = rows_alloc - 63;  // rows_alloc >= 64
+  return ax;
+}
+
+// Intensity at local axis position t (cubic Lagrange over 4 rows or linear
+// over 2, each row contracted over 4 mu columns; weights already carry
+// flux/I).  `rows` points at the staged slice row 0 plus the patch's first mu
+// column.
+template <int EM>
 __device__ inline float nsx_eval(float t, const EAxis& ax, const float* rows, const float w[4]) {
   t = __builtin_amdgcn_fmed3f(t, ax.t_lo, ax.t_hi);
-  const int st = med3i((int)floorf(t) - 1, ax.st_lo, ax.st_hi);
+  const int f = med3i((int)floorf(t), ax.f_lo, ax.f_hi);
+  if (EM == kELinear) {
+    const float* r0 = rows + __mul24(f, kSliceStride);
+    const float* r1 = r0 + kSliceStride;
+    const float m0 = w[0] * r0[0] + w[1] * r0[1] + w[2] * r0[2] + w[3] * r0[3];
+    const float m1 = w[0] * r1[0] + w[1] * r1[1] + w[2] * r1[2] + w[3] * r1[3];
+    return fmaf(t - (float)f, m1 - m0, m0);
+  }
+  const int st = f - 1;
   const float v = t - (float)st;
   const float d1 = v - 1.f, d2 = v - 2.f, d3 = v - 3.f;
   const float p23 = d2 * d3, p01 = v * d1;
@@ -165,44 +199,50 @@ __device__ inline bool energies_ascend(const float (&le)[EPL], int lane) {
 
 // All energies of a wavefront at one patch (uniform tx, im, w).  The patch's
 // mu stencil is contracted once per slice row the wavefront touches (one row
-// per lane), each row start gets the Newton coefficients of the cubic
-// through 4 consecutive rows,
-//   I(v) = r0 + v (D1 + (v-1) (D2/2 + (v-2) D3/6)),  v = t - st in [0, 3],
-// which is the cubic Lagrange polynomial of cpu/nsx.hpp:159-167 over those
-// rows, and each energy reads one coefficient set.  Requires energies that
-// ascend with (u, lane) (energies_ascend); returns false, with nothing
-// written, when the rows exceed one per lane (the caller then uses nsx_eval).
-template <int EPL>
+// per lane), and each row f gets the coefficients of the polynomial from f:
+// cubic (rows f-1..f+2, Newton form in v = t - (f - 1) in [0, 3])
+//   I(v) = r0 + v (D1 + (v-1) (D2/2 + (v-2) D3/6)),
+// the cubic Lagrange polynomial of cpu/nsx.hpp:159-167 over those rows, or
+// linear (rows f, f+1), I = r0 + x (r1 - r0), x = t - f; each energy reads one
+// coefficient set.  Requires energies that ascend with (u, lane)
+// (energies_ascend); returns false, with nothing written, when the rows
+// exceed one per lane (the caller then uses nsx_eval).
+template <int EPL, int EM>
 __device__ inline bool nsx_eval_wave(const float (&le)[EPL], float tx, int im, const float (&w)[4], const EAxis& ax,
                                      const float* s_S, float4* s_C, int lane, float (&out)[EPL]) {
+  constexpr int kSpan = EM == kELinear ? 1 : 3;  // rows beyond a coefficient set's first
   int st[EPL];
   float v[EPL];
 #pragma unroll
   for (int u = 0; u < EPL; ++u) {
     const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
-    st[u] = med3i((int)floorf(t) - 1, ax.st_lo, ax.st_hi);
+    st[u] = med3i((int)floorf(t), ax.f_lo, ax.f_hi) - (EM == kELinear ? 0 : 1);
     v[u] = t - (float)st[u];
   }
   const int r_lo = readlane_i(st[0], 0);
   const int n_c = readlane_i(st[EPL - 1], 63) - r_lo + 1;  // coefficient sets needed
-  if (n_c > 61) return false;
+  if (n_c > 64 - kSpan) return false;
   float U = 0.f;
-  if (lane < n_c + 3) {
+  if (lane < n_c + kSpan) {
     const float* p = s_S + __mul24(r_lo + lane, kSliceStride) + im;
     U = w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * p[3];
   }
   const float U1 = __shfl_down(U, 1), U2 = __shfl_down(U, 2), U3 = __shfl_down(U, 3);
   if (lane < n_c) {
-    const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
-    const float d2 = d1b - d1, d3 = (d1c - d1b) - d2;
-    s_C[lane] = make_float4(U, d1, 0.5f * d2, (1.f / 6.f) * d3);
+    if (EM == kELinear) {
+      s_C[lane] = make_float4(U, U1 - U, 0.f, 0.f);
+    } else {
+      const float d1 = U1 - U, d1b = U2 - U1, d1c = U3 - U2;
+      const float d2 = d1b - d1, d3 = (d1c - d1b) - d2;
+      s_C[lane] = make_float4(U, d1, 0.5f * d2, (1.f / 6.f) * d3);
+    }
   }
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int u = 0; u < EPL; ++u) {
     const float4 k = s_C[med3i(st[u] - r_lo, 0, 63)];
     const float x = v[u];
-    out[u] = fmaf(x, fmaf(x - 1.f, fmaf(x - 2.f, k.w, k.z), k.y), k.x);
+    out[u] = EM == kELinear ? fmaf(x, k.y, k.x) : fmaf(x, fmaf(x - 1.f, fmaf(x - 2.f, k.w, k.z), k.y), k.x);
   }
   __builtin_amdgcn_wave_barrier();
   return true;
@@ -236,11 +276,11 @@ __device__ inline WaveEnergies<SETS> wave_energies(const float (&le)[EPL], int l
 // Classification of one patch, made once in its set-up lane.  When the
 // wavefront's energies allow nsx_eval_rows (WaveEnergies::fast) the patch is
 // evaluated there: without clamps when no energy reaches a clamp of the axis
-// or of the stencil start (checked on the first and last energy of each set,
-// the evaluation being monotone in the energy), with them otherwise.
-// Set s holds slice rows i0[s] - 1 + lane: i0[s] is at most the stencil
-// start + 1 of the set's first energy, and at most EAxis::i0_max so that
-// every lane's row is in the staged allocation.  txs = tx - i0 puts a set's
+// or of the row f (checked on the first and last energy of each set, the
+// evaluation being monotone in the energy), with them otherwise.
+// Set s holds slice rows i0[s] - 1 + lane: i0[s] is at most the row f of the
+// set's first energy, and at most EAxis::i0_max so that every lane's row is
+// in the staged allocation.  txs = tx - i0 puts a set's
 // energies on its lane axis (floor(le + txs) is the lane holding the row),
 // rb = (i0 - 1) * kSliceStride + im is the LDS offset of lane 0's row at the
 // patch's first mu column.
@@ -259,7 +299,7 @@ __device__ inline PatchRows<SETS> patch_rows(float tx, int im, const WaveEnergie
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
     const float tc = __builtin_amdgcn_fmed3f(we.le_first[s] + tx, ax.t_lo, ax.t_hi);
-    r.i0[s] = min(med3i((int)floorf(tc), ax.st_lo + 1, ax.st_hi + 1), ax.i0_max);
+    r.i0[s] = min(med3i((int)floorf(tc), ax.f_lo, ax.f_hi), ax.i0_max);
     r.txs[s] = tx - (float)r.i0[s];
     r.rb[s] = (r.i0[s] - 1) * kSliceStride + im;
     // the unclamped path evaluates at le + txs (rounded once): its end
@@ -267,7 +307,7 @@ __device__ inline PatchRows<SETS> patch_rows(float tx, int im, const WaveEnergie
     // (the rounding of le + tx may differ near an integer)
     const float uf = we.le_first[s] + r.txs[s], ul = we.le_last[s] + r.txs[s];
     const float i0f = (float)r.i0[s];
-    r.interior = r.interior && uf >= 0.f && ul < (float)(ax.st_hi + 2) - i0f && ul <= ax.t_hi - i0f;
+    r.interior = r.interior && uf >= 0.f && ul < (float)(ax.f_hi + 1) - i0f && ul <= ax.t_hi - i0f;
   }
   return r;
 }
@@ -287,18 +327,21 @@ __device__ inline int floor_to_int(float t) {
 // Contracted evaluation of NP patches at once (independent chains, for
 // latency hiding), every patch of a wavefront with WaveEnergies::fast and,
 // with CLAMP = false, PatchRows::interior; patch j is the one set up in lane
-// q[j] (pv, pr).  For energy t = le + tx (clamped to the axis ends) the
-// stencil start is st = floor(t) - 1 (clamped to the legal starts) and
-// x = t - (st + 1), in [0, 1) away from the clamps.  For each row set, lane
-// l contracts slice row i0 - 1 + l over the patch's mu stencil, takes the
-// next three rows from its neighbours and forms the power-basis coefficients
-// of the cubic through the four rows at the nodes x = -1, 0, 1, 2:
+// q[j] (pv, pr).  For energy t = le + tx (clamped to the axis ends) the row
+// is f = floor(t) (clamped to EAxis::f_lo..f_hi) and x = t - f, in [0, 1)
+// away from the clamps.  For each row set, lane l contracts slice row
+// i0 - 1 + l over the patch's mu stencil, takes the next rows from its
+// neighbours and forms the power-basis coefficients of the polynomial in x of
+// row f = i0 + l: the cubic through rows f-1..f+2 at the nodes x = -1, 0, 1, 2,
 //   I(x) = a0 + x (a1 + x (a2 + x a3))
-// (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256); each
+// (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256), or the
+// line through rows f, f+1, I(x) = a0 + x a1 (cpu/nsx.hpp:304-316); each
 // energy then fetches the coefficients of its row from the lane that holds
 // them.  The differences of lane l + 1 are the next differences of lane l,
 // so each order takes one shifted subtraction.  Away from the clamps an
-// energy's lane and x come from t - i0 = le + txs directly.
+// energy's lane and x come from t - i0 = le + txs directly.  EM = kECubicClip
+// sets negative intensities to 0 (use_sg, cpu/nsx.hpp:201; the weights carry
+// the positive flux/I x weight, so the sign is the interpolant's).
 //
 // nsx_load_rows issues the slice-row reads of patch q (lane l: row i0 - 1 + l
 // of each set at the patch's four mu columns); a caller may issue them for
@@ -314,11 +357,12 @@ __device__ inline void nsx_load_rows(const PatchRows<SETS>& pr, int q, const flo
   }
 }
 
-template <int EPL, int NP, bool CLAMP, int SETS>
+template <int EPL, int NP, bool CLAMP, int SETS, int EM>
 __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
                                        const int (&q)[NP], const EAxis& ax, const float (&p)[NP][SETS][4], int lane,
                                        float (&out)[NP][EPL]) {
   constexpr int G = EPL / SETS;
+  constexpr bool kLin = EM == kELinear;
   float a0[NP][SETS], a1[NP][SETS], a2[NP][SETS], a3[NP][SETS];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -329,13 +373,19 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
     for (int s = 0; s < SETS; ++s) {
       const float U = w[0] * p[j][s][0] + w[1] * p[j][s][1] + w[2] * p[j][s][2] + w[3] * p[j][s][3];
       const float U1 = lane_next(U);
-      const float d1 = U1 - U;                // U(0) - U(-1)
-      const float D2 = lane_next(d1) - d1;    // second difference at x = 0
-      const float D3 = lane_next(D2) - D2;    // third difference
-      a2[j][s] = 0.5f * D2;
-      a3[j][s] = (1.f / 6.f) * D3;
-      a1[j][s] = d1 + a2[j][s] - a3[j][s];
-      a0[j][s] = U1;
+      if (kLin) {
+        a0[j][s] = U1;                   // row f
+        a1[j][s] = lane_next(U1) - U1;   // row f+1 - row f
+        a2[j][s] = a3[j][s] = 0.f;
+      } else {
+        const float d1 = U1 - U;                // U(0) - U(-1)
+        const float D2 = lane_next(d1) - d1;    // second difference at x = 0
+        const float D3 = lane_next(D2) - D2;    // third difference
+        a2[j][s] = 0.5f * D2;
+        a3[j][s] = (1.f / 6.f) * D3;
+        a1[j][s] = d1 + a2[j][s] - a3[j][s];
+        a0[j][s] = U1;
+      }
     }
   }
   float x[NP][EPL], c[NP][EPL][4];
@@ -358,9 +408,9 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
       int src;
       if (CLAMP) {
         const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
-        const int f1 = med3i(floor_to_int(t), ax.st_lo + 1, ax.st_hi + 1);
-        x[j][u] = t - (float)f1;
-        src = (f1 - i0[s]) << 2;
+        const int f = med3i(floor_to_int(t), ax.f_lo, ax.f_hi);
+        x[j][u] = t - (float)f;
+        src = (f - i0[s]) << 2;
       } else {
         const float t = le[u] + txs[s];
         src = floor_to_int(t) << 2;
@@ -368,18 +418,26 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
       }
       c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j][s])));
       c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j][s])));
-      c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j][s])));
-      c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j][s])));
+      if (!kLin) {
+        c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j][s])));
+        c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j][s])));
+      }
     }
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j)
 #pragma unroll
-    for (int u = 0; u < EPL; ++u)
-      out[j][u] = fmaf(x[j][u], fmaf(x[j][u], fmaf(x[j][u], c[j][u][3], c[j][u][2]), c[j][u][1]), c[j][u][0]);
+    for (int u = 0; u < EPL; ++u) {
+      float r;
+      if (kLin)
+        r = fmaf(x[j][u], c[j][u][1], c[j][u][0]);
+      else
+        r = fmaf(x[j][u], fmaf(x[j][u], fmaf(x[j][u], c[j][u][3], c[j][u][2]), c[j][u][1]), c[j][u][0]);
+      out[j][u] = EM == kECubicClip ? fmaxf(r, 0.f) : r;
+    }
 }
 
-template <int EPL, int NP, bool CLAMP, int SETS>
+template <int EPL, int NP, bool CLAMP, int SETS, int EM>
 __device__ inline void nsx_eval_rows(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
                                      const int (&q)[NP], const EAxis& ax, const float* s_S, int lane,
                                      float (&out)[NP][EPL]) {
@@ -387,41 +445,64 @@ __device__ inline void nsx_eval_rows(const float (&le)[EPL], const PatchVals& pv
   float p[NP][SETS][4];
 #pragma unroll
   for (int j = 0; j < NP; ++j) nsx_load_rows(pr, q[j], s_S, lane, p[j]);
-  nsx_eval_loaded<EPL, NP, CLAMP, SETS>(le, pv, pr, q, ax, p, lane, out);
+  nsx_eval_loaded<EPL, NP, CLAMP, SETS, EM>(le, pv, pr, q, ax, p, lane, out);
 }
 
 // One patch whose wavefront energies span too many rows for nsx_eval_rows:
 // contracted evaluation with clamps (nsx_eval_wave) when they fit the
 // scratch, else per energy (nsx_eval).
-template <int EPL>
+template <int EPL, int EM>
 __device__ inline void nsx_eval_edge(const float (&le)[EPL], bool asc, float tx, int im, const float (&w)[4],
                                      const EAxis& ax, const float* s_S, float4* s_C, int lane, float (&out)[EPL]) {
-  if (!(asc && nsx_eval_wave<EPL>(le, tx, im, w, ax, s_S, s_C, lane, out))) {
+  if (!(asc && nsx_eval_wave<EPL, EM>(le, tx, im, w, ax, s_S, s_C, lane, out))) {
 #pragma unroll
     for (int u = 0; u < EPL; ++u) {
-      out[u] = nsx_eval(le[u] + tx, ax, s_S + im, w);
+      out[u] = nsx_eval<EM>(le[u] + tx, ax, s_S + im, w);
       __builtin_amdgcn_sched_barrier(0);  // rare path: keep its register use low
     }
   }
+  if (EM == kECubicClip)
+#pragma unroll
+    for (int u = 0; u < EPL; ++u) out[u] = fmaxf(out[u], 0.f);
 }
 
-// Stage the rows [r_lo, r_lo + n_rows) of the (logT, log g) bilinear blend of
-// the NSX grid into LDS (row stride kSliceStride, mu padding zeroed).
-__device__ inline void stage_nsx_rows(const DevNsx& N, int iT, int ig, float aT, float ag, int r_lo, int n_rows,
-                                      float* s_S, int tid, int nthreads) {
-  const float w00 = (1.f - aT) * (1.f - ag), w01 = (1.f - aT) * ag, w10 = aT * (1.f - ag), w11 = aT * ag;
+// Stage the rows [r_lo, r_lo + n_rows) of the (logT, log g) blend of the NSX
+// grid into LDS (row stride kSliceStride, mu padding zeroed): the ring's
+// 2 x 2 (bilinear) or 4 x 4 (bicubic Lagrange) stencil with the weights
+// k_ring_setup computed in FP64.
+__device__ inline void stage_nsx_rows(const DevNsx& N, const RingDesc& D, int r_lo, int n_rows, float* s_S, int tid,
+                                      int nthreads) {
   const size_t sT = (size_t)N.ng * N.nE * N.nm, sg = (size_t)N.nE * N.nm;
-  const float* c00 = N.data + iT * sT + ig * sg + (size_t)r_lo * N.nm;
-  const float* c01 = c00 + sg;
-  const float* c10 = c00 + sT;
-  const float* c11 = c10 + sg;
+  const float* c00 = N.data + D.iT * sT + D.ig * sg + (size_t)r_lo * N.nm;
   const int total = n_rows * kSliceStride;
+  if (D.n_tg == 2) {
+    const float w00 = D.wtg[0], w01 = D.wtg[1], w10 = D.wtg[4], w11 = D.wtg[5];
+    const float* c01 = c00 + sg;
+    const float* c10 = c00 + sT;
+    const float* c11 = c10 + sg;
+    for (int k = tid; k < total; k += nthreads) {
+      const int r = k / kSliceStride, m = k - r * kSliceStride;
+      float v = 0.f;
+      if (m < N.nm) {
+        const size_t o = (size_t)r * N.nm + m;
+        v = w00 * c00[o] + w01 * c01[o] + w10 * c10[o] + w11 * c11[o];
+      }
+      s_S[k] = v;
+    }
+    return;
+  }
+  float w[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = D.wtg[i];
   for (int k = tid; k < total; k += nthreads) {
     const int r = k / kSliceStride, m = k - r * kSliceStride;
     float v = 0.f;
     if (m < N.nm) {
-      const size_t o = (size_t)r * N.nm + m;
-      v = w00 * c00[o] + w01 * c01[o] + w10 * c10[o] + w11 * c11[o];
+      const float* c = c00 + (size_t)r * N.nm + m;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v = fmaf(w[4 * a + b], c[a * sT + b * sg], v);
     }
     s_S[k] = v;
   }

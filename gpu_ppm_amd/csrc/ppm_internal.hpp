@@ -21,6 +21,23 @@ constexpr double kPlanckErgS = 6.62607015E-27;
 constexpr double kBoltzKeVK = 8.617333262E-8;
 constexpr double kGMsunCgs = 1.32712440041279419E16;
 
+// The NSX interpolants (PPM_NSX_*) in three parts: the energy-axis
+// evaluation (a kernel template parameter), the mu stencil and the
+// (logT, log g) stencil (both per spot).
+enum NsxEnergyMode { kECubic = 0, kECubicClip = 1, kELinear = 2 };
+enum NsxMuMode { kMuCubicLinearEdge = 0, kMuCubic = 1, kMuLinear = 2 };
+constexpr int nsx_energy_mode(int interp) {
+  return interp == PPM_NSX_4L ? kELinear : interp == PPM_NSX_4C_SG ? kECubicClip : kECubic;
+}
+constexpr int nsx_mu_mode(int interp) {
+  return interp == PPM_NSX_4L ? kMuLinear
+         : (interp == PPM_NSX_4C || interp == PPM_NSX_4C_SG) ? kMuCubic
+                                                             : kMuCubicLinearEdge;
+}
+constexpr bool nsx_cubic_tg(int interp) {
+  return interp == PPM_NSX_4C || interp == PPM_NSX_4C_SG || interp == PPM_NSX_4C_LE;
+}
+
 constexpr int kMaxRuns = 16;      // active-pixel runs per ring kept in a descriptor
 constexpr int kSliceStride = 69;  // LDS row stride of the NSX slice (67 mu + 2 zero pad)
 
@@ -37,7 +54,7 @@ struct SpotParams {
   double kT, inv_kT, logT, kT_nsx;   // temperatures
   double scale;                      // FP64 constant applied at the reduction
   int n_side;
-  int pad;
+  int nsx_interp;                    // PPM_NSX_*
 };
 
 // One candidate ring of one spot (input of the setup kernel).
@@ -55,8 +72,10 @@ struct RingDesc {
   int n_act, n_runs;          // active patches and runs
   int kept;                   // 1 if the ring contributes
   int flags;                  // bit0: run overflow, bit1: NSX rows beyond the staged slice
-  int iT, ig;                 // NSX (logT, logg) bracket
-  float aT, ag;
+  int iT, ig;                 // first (logT, log g) node of the NSX stencil
+  int n_tg;                   // stencil nodes per axis: 2 (linear) or 4 (cubic)
+  int pad;
+  float wtg[16];              // weight of node (iT + a, ig + b) at [4 a + b]
   int run_start[kMaxRuns];    // first active pixel j (1-based) of each run
   int run_len[kMaxRuns];
 };
@@ -103,6 +122,7 @@ struct PlainLaunch {
   double* d_flux;        // [n_phase][n_e] output (one star)
   int n_tasks, n_fine, n_pts, n_phase, n_e, n_echunks;
   int atmosphere;
+  int emode;             // NsxEnergyMode of the star's interpolant
   int nsx_rows;          // NSX slice rows staged per workgroup
 };
 

@@ -69,6 +69,34 @@ __device__ inline int uniform_hunt(double x_min, double x_max, int n, double x, 
   return i;
 }
 
+// Stencil of one NSX table axis at x (FP64): the bracket's two linear
+// weights (cpu/nsx.hpp:277-280), or the 4-point Lagrange weights of the nodes
+// from one below the bracket, clamped to [0, n-4], at x clamped to the axis
+// (cpu/nsx.hpp:159-167, 172-186).  Returns the first node.
+__device__ inline int axis_stencil(double x_min, double x_max, int n, double x, int n_w, double* w) {
+  double a;
+  const int i = uniform_hunt(x_min, x_max, n, x, a);
+  if (n_w == 2) {
+    w[0] = 1 - a;
+    w[1] = a;
+    return i;
+  }
+  x = fmin(fmax(x, x_min), x_max);
+  const int s = clampi(i - 1, 0, n - 4);
+  double xs[4];
+  for (int k = 0; k < 4; ++k) xs[k] = x_min + (x_max - x_min) * (s + k) / (n - 1);  // cpu/hunt.hpp:180-187
+  for (int k = 0; k < 4; ++k) {
+    double num = 1, den = 1;
+    for (int l = 0; l < 4; ++l)
+      if (l != k) {
+        num *= x - xs[l];
+        den *= xs[k] - xs[l];
+      }
+    w[k] = num / den;
+  }
+  return s;
+}
+
 __device__ inline void lens_ca_lf(const DevLensing& L, double u, double cp, double& ca, double& lf) {
   double au, ac;  // cpu/lensing.hpp:43-60
   const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, au);
@@ -390,16 +418,17 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
       logg = log10(g_g0) + S.logg0_total;
     }
     D->logg = logg;
+    double wT[4] = {0, 0, 0, 0}, wg[4] = {0, 0, 0, 0};
+    D->iT = D->ig = 0;
+    D->n_tg = 2;
     if (N.data) {
-      double aT, ag;
-      D->iT = uniform_hunt(N.logT_min, N.logT_max, N.nT, S.logT, aT);
-      D->ig = uniform_hunt(N.logg_min, N.logg_max, N.ng, logg, ag);
-      D->aT = (float)aT;
-      D->ag = (float)ag;
-    } else {
-      D->iT = D->ig = 0;
-      D->aT = D->ag = 0.f;
+      const int n_tg = nsx_cubic_tg(S.nsx_interp) ? 4 : 2;
+      D->iT = axis_stencil(N.logT_min, N.logT_max, N.nT, S.logT, n_tg, wT);
+      D->ig = axis_stencil(N.logg_min, N.logg_max, N.ng, logg, n_tg, wg);
+      D->n_tg = n_tg;
     }
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) D->wtg[4 * a + b] = (float)(wT[a] * wg[b]);
     D->kept = 1;
   }
 }
@@ -440,7 +469,7 @@ constexpr int kNP = 4;                      // patches evaluated together
 // phase bins round-robin.  Within a wavefront each lane sets up one patch in
 // FP64 and keeps it in registers; the evaluation loop broadcasts a patch
 // with readlane and every lane evaluates it at its two energies.
-template <int ATM>
+template <int ATM, int EM>
 __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx N, PlainLaunch P) {
   const int task = blockIdx.x, chunk = blockIdx.y;
   RingDesc* Dp = P.d_desc + task;
@@ -490,13 +519,9 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
       if (tid == 0) atomicOr(&Dp->flags, 2);
       n_rows = rows_alloc;
     }
-    ax.t_lo = (float)(-r_lo);
-    ax.t_hi = (float)(N.nE - 1 - r_lo);
-    ax.st_lo = 0;
-    ax.st_hi = min(N.nE - 4 - r_lo, n_rows - 4);
-    ax.i0_max = rows_alloc - 63;  // rows_alloc >= 64
+    ax = energy_axis<EM>(N.nE, r_lo, n_rows, rows_alloc);
     for (int k = tid; k < N.nm; k += kBlock) s_mu[k] = N.mu[k];
-    stage_nsx_rows(N, Dp->iT, Dp->ig, Dp->aT, Dp->ag, r_lo, n_rows, s_S, tid, kBlock);
+    stage_nsx_rows(N, *Dp, r_lo, n_rows, s_S, tid, kBlock);
   }
   __syncthreads();
   const TxMap txm{(float)((l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo),
@@ -520,6 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
   const double ps_shift = Dp->ps_shift;
   const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
   const int ng = P.n_pts / P.n_phase;
+  const int mu_mode = nsx_mu_mode(S.nsx_interp);
 
   for (int b = wave; b < P.n_phase; b += kWaves) {
     float acc[kEPL], comp[kEPL];
@@ -552,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
           const double v = pt.p + shift + 1.;
           double pp = v - floor(v);
           if (pp < s_ph[0]) pp += 1.;
-          patch_setup<ATM>(pp, pt.w, R, s_mu, N.mu_den, N.nm, N.mu_min, txm, pv);
+          patch_setup<ATM>(pp, pt.w, R, s_mu, N.mu_den, N.nm, N.mu_min, mu_mode, txm, pv);
         }
         // ---- evaluation: every lane evaluates each visible patch at its
         // energies, kNP patches at a time
@@ -579,9 +605,9 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
             float val[kNP][kEPL];
             if (we.fast) {
               if (all_interior)
-                nsx_eval_rows<kEPL, kNP, false, 1>(le, pv, pr, q, ax, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, false, 1, EM>(le, pv, pr, q, ax, s_S, lane, val);
               else
-                nsx_eval_rows<kEPL, kNP, true, 1>(le, pv, pr, q, ax, s_S, lane, val);
+                nsx_eval_rows<kEPL, kNP, true, 1, EM>(le, pv, pr, q, ax, s_S, lane, val);
             } else {
 #pragma unroll
               for (int j = 0; j < kNP; ++j) {
@@ -589,8 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
                 float w[4];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[j]);
-                nsx_eval_edge<kEPL>(le, asc, readlane_f(pv.tx, q[j]), readlane_i(pv.im, q[j]), w, ax, s_S, s_C, lane,
-                                    val[j]);
+                nsx_eval_edge<kEPL, EM>(le, asc, readlane_f(pv.tx, q[j]), readlane_i(pv.im, q[j]), w, ax, s_S, s_C,
+                                        lane, val[j]);
               }
             }
 #pragma unroll
@@ -795,22 +821,28 @@ int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P
   return hipGetLastError();
 }
 
+template <int ATM, int EM>
+static hipError_t launch_plain_inst(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, dim3 grid, size_t lds,
+                                    hipStream_t st) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<ATM, EM>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_integral_plain<ATM, EM>), grid, dim3(kBlock), lds, st, L, N, P);
+  return hipGetLastError();
+}
+
 int launch_integral_plain(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream) {
   if (P.n_tasks == 0) return hipSuccess;
   const size_t lds = integral_lds_bytes(P.n_fine, P.atmosphere, P.nsx_rows);
-  dim3 grid(P.n_tasks, P.n_echunks);
-  if (P.atmosphere == PPM_ATM_NSX) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<PPM_ATM_NSX>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_integral_plain<PPM_ATM_NSX>, grid, dim3(kBlock), lds, (hipStream_t)stream, L, N, P);
-  } else {
-    hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<PPM_ATM_BLACKBODY>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_integral_plain<PPM_ATM_BLACKBODY>, grid, dim3(kBlock), lds, (hipStream_t)stream, L, N, P);
+  const dim3 grid(P.n_tasks, P.n_echunks);
+  const hipStream_t st = (hipStream_t)stream;
+  if (P.atmosphere != PPM_ATM_NSX) return launch_plain_inst<PPM_ATM_BLACKBODY, kECubic>(L, N, P, grid, lds, st);
+  switch (P.emode) {
+    case kECubic: return launch_plain_inst<PPM_ATM_NSX, kECubic>(L, N, P, grid, lds, st);
+    case kECubicClip: return launch_plain_inst<PPM_ATM_NSX, kECubicClip>(L, N, P, grid, lds, st);
+    case kELinear: return launch_plain_inst<PPM_ATM_NSX, kELinear>(L, N, P, grid, lds, st);
   }
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
 
 int reduce_groups(int n_tasks) { return n_tasks < 32 ? 1 : (n_tasks / 16 < 64 ? n_tasks / 16 : 64); }

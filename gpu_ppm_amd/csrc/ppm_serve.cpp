@@ -3,7 +3,8 @@
 // ppm_submit validates a request, copies its star, grid and energy grid,
 // charges its device memory against the context's budget and queues it.
 // A worker thread owned by the context takes, whenever the device is free,
-// every queued request whose atmosphere is the oldest request's (up to
+// every queued request whose launch class (atmosphere and, for NSX, the
+// interpolant's energy-axis evaluation) is the oldest request's (up to
 // kMaxBatch, in arrival order), groups them by grid and evaluates all of
 // them in one persistent batched launch (several grids per launch), while
 // new requests keep arriving.  Results are copied to host memory of the
@@ -48,7 +49,7 @@ bool same_grid(const Request& a, const Request& b) {
   const ppm_grid &x = a.grid, &y = b.grid;
   return x.n_side == y.n_side && x.n_phase == y.n_phase && x.sampling == y.sampling &&
          (x.sampling != PPM_SAMPLE_GL || x.n_gl == y.n_gl) && x.n_fine == y.n_fine && x.response == y.response &&
-         a.energies == b.energies && a.star.atmosphere == b.star.atmosphere;
+         a.energies == b.energies && batch_class(a.star) == batch_class(b.star);
 }
 
 }  // namespace
@@ -83,11 +84,11 @@ void worker_loop(ppm_ctx* c) {
       std::unique_lock<std::mutex> lk(S.mu);
       S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty(); });
       if (S.stop) return;
-      // the oldest request and every queued request of its atmosphere
-      const int atm = S.reqs.at(S.queue.front())->star.atmosphere;
+      // the oldest request and every queued request of its launch class
+      const int cls = batch_class(S.reqs.at(S.queue.front())->star);
       for (auto it = S.queue.begin(); it != S.queue.end() && (int)batch.size() < kMaxBatch;) {
         Request* r = S.reqs.at(*it).get();
-        if (r->star.atmosphere == atm) {
+        if (batch_class(r->star) == cls) {
           r->state = ReqState::kRunning;
           batch.push_back(r);
           tickets.push_back(*it);

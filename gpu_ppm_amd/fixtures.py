@@ -10,6 +10,8 @@ Layout under a root directory (what the drop-in programs read):
   tables/ism_inst/std128  300 channels x 128 E_obs, seed 0
   ext/model_data/nsx_H_v200804.out   35x14x166x67 NSX-format grid, seed 1
   ext/model_data/nsx_small.out       6x5x166x67 grid on the same axes, seed 1
+  ext/model_data/nsx_H_v171019.out   35x11x166x67 grid on the nsx_H_v171019
+                                     axes (log g 13.7-14.7), seed 1
 """
 import os
 import subprocess
@@ -60,6 +62,15 @@ def ensure_nsx(root, small=False):
             _synth("nsx", p, 1, *SMALL_NSX_DIMS)
         else:
             _synth("nsx", p, 1)
+    return p
+
+
+def ensure_nsx_v171019(root):
+    """NSX-format grid on the nsx_H_v171019 axes (35x11x166x67, log g
+    13.7-14.7, cpu/nsx.hpp:39-51), seed 1."""
+    p = os.path.join(root, "ext", "model_data", "nsx_H_v171019.out")
+    if not os.path.exists(p):
+        _synth("nsx", p, 1, 35, 11, 166, 67, 14.7)
     return p
 
 

@@ -4,6 +4,7 @@ benchmark configurations, expressed as ppm_star / ppm_grid values.
 Reference scenarios:
   os1 a..j        cpu/os1_cpu.cpp:17-36 (N_side 256, 128 phase points, 1 keV)
   devarshi stars  cpu/devarshi24st_cpu.cpp:17-22, :48-59, :267-285
+  t_interp tests  cpu/t_interp.cpp:19-33, :242-261 (NSX interpolants)
 """
 import math
 
@@ -76,6 +77,48 @@ def devarshi_star(name, kT_keV=DEVARSHI_KT_KEV):
 def devarshi_grid(n_side, response=0, n_phase=DEVARSHI_N_PHASE, method=abi.METHOD_AUTO):
     return abi.make_grid(n_side, n_phase, n_fine=6 * n_phase, sampling=abi.SAMPLE_GL, n_gl=2,
                          energies=None, response=response, method=method)
+
+
+# ------------------------------------------------------------- t_interp
+TINTERP = {  # name: (inclination, spot colatitude, interp_type), cpu/t_interp.cpp:248-261
+    "test10": (0.01, 1.983, 0), "test11": (0.01, 1.983, 1),
+    "test12": (0.01, 1.983, 2), "test13": (0.01, 1.983, 3),
+    "test20": (0.964, math.pi - 0.01, 0), "test21": (0.964, math.pi - 0.01, 1),
+    "test22": (0.964, math.pi - 0.01, 2), "test23": (0.964, math.pi - 0.01, 3),
+}
+# interp_type -> interpolant (cpu/t_interp.cpp:210-219)
+TINTERP_MODE = {0: abi.NSX_4C, 1: abi.NSX_4C_SG, 2: abi.NSX_4C_LE, 3: abi.NSX_2L2C_LE}
+TINTERP_N_PHASE = 256
+TINTERP_N_FINE = 1024
+
+
+def ref_linspace(beg, end, n):
+    """linspace of cpu/ut.hpp:7-14 (beg + i * step)."""
+    step = (end - beg) / (n - 1)
+    return np.array([beg + i * step for i in range(n)])
+
+
+def tinterp_energies():
+    return ref_linspace(0.1, 5.0, 1024)  # cpu/t_interp.cpp:33
+
+
+def tinterp_star(name, nsx_interp=None):
+    """One 0.01 rad spot (a point-source element, cpu/grid.hpp:17-27) at
+    logT 6, M 1.4, R 12 km, 600 Hz, D 0.15 kpc; log g from the spot's
+    colatitude (cpu/t_interp.cpp:20-58, 76-97)."""
+    inc, th, it = TINTERP[name]
+    spot = abi.make_spot(th, 0.01, DEVARSHI_KT_KEV, membership=abi.MEMBER_CAP)
+    return abi.make_star([spot], mass_msun=1.4, radius_eq_km=12.0, spin_hz=600.0, inclination_rad=inc,
+                         distance_km=0.15 * abi.KPC_KM, point_source_below=0.1, atmosphere=abi.ATM_NSX,
+                         variant=abi.VARIANT_DEVARSHI,
+                         nsx_interp=TINTERP_MODE[it] if nsx_interp is None else nsx_interp)
+
+
+def tinterp_grid(energies=None, method=abi.METHOD_AUTO):
+    """256 bins x 2 GL nodes, 1024 fine phases (cpu/t_interp.cpp:26, 245-246, 188-196)."""
+    return abi.make_grid(16, TINTERP_N_PHASE, n_fine=TINTERP_N_FINE, sampling=abi.SAMPLE_GL, n_gl=2,
+                         energies=tinterp_energies() if energies is None else energies, response=-1,
+                         method=method)
 
 
 # ------------------------------------------------- benchmark configurations

@@ -13,10 +13,11 @@
  *       uniform cos(psi) axis.  Quadrature here is adaptive Gauss-Kronrod
  *       7/15 (QUADPACK's rule) instead of scipy's QAGS.
  *
- *   nsx      PATH [seed]
+ *   nsx      PATH [seed nT ng nE nmu [logg_max]]
  *       NSX-format text grid ("logEkT mu logInuT3 logT logg", loop order
- *       logT > logg > logEkT > mu, mu decreasing; cpu/nsx.hpp:83-148) of the
- *       35x14x166x67 nsx_H_v200804 shape, filled with a limb-darkened Planck
+ *       logT > logg > logEkT > mu, mu decreasing; cpu/nsx.hpp:83-148), by
+ *       default of the 35x14x166x67 nsx_H_v200804 shape (log g 13.7-15.0;
+ *       nsx_H_v171019 is 35 11 166 67 with logg_max 14.7), filled with a limb-darkened Planck
  *       intensity B_nu(T)/T^3 (1 + a mu) (1 + 0.02 n), n a seeded smooth
  *       noise with |n| <= 1.
  *
@@ -327,8 +328,8 @@ static int cmd_lensing(const char* dir, double u_min, double u_max, int n_u, int
 
 /* ------------------------------------------------------------ NSX grid */
 
-static int cmd_nsx(const char* path, uint64_t seed, int nT, int ng, int nE, int nm) {
-  const double logT_min = 5.10, logT_max = 6.80, logg_min = 13.7, logg_max = 15.0;
+static int cmd_nsx(const char* path, uint64_t seed, int nT, int ng, int nE, int nm, double logg_max) {
+  const double logT_min = 5.10, logT_max = 6.80, logg_min = 13.7;
   const double logE_min = -1.30, logE_max = 2.00; /* cpu/nsx.hpp:25-37 */
   const double kB = 1.380649e-16, h = 6.62607015e-27, c = 2.99792458e10;
   const double pref = 2.0 * kB * kB * kB / (h * h * c * c);
@@ -458,7 +459,8 @@ int main(int argc, char** argv) {
     uint64_t seed = argc > 3 ? strtoull(argv[3], 0, 10) : 1;
     int nT = argc > 4 ? atoi(argv[4]) : 35, ng = argc > 5 ? atoi(argv[5]) : 14;
     int nE = argc > 6 ? atoi(argv[6]) : 166, nm = argc > 7 ? atoi(argv[7]) : 67;
-    return cmd_nsx(argv[2], seed, nT, ng, nE, nm);
+    double logg_max = argc > 8 ? atof(argv[8]) : 15.0;
+    return cmd_nsx(argv[2], seed, nT, ng, nE, nm, logg_max);
   }
   if (argc >= 5 && !strcmp(argv[1], "response")) {
     uint64_t seed = argc > 5 ? strtoull(argv[5], 0, 10) : 0;
@@ -466,7 +468,7 @@ int main(int argc, char** argv) {
   }
   fprintf(stderr,
           "usage: ppm_synth lensing OUTDIR [u_min u_max n_u n_cos]\n"
-          "       ppm_synth nsx PATH [seed nT ng nE nmu]\n"
+          "       ppm_synth nsx PATH [seed nT ng nE nmu [logg_max]]\n"
           "       ppm_synth response OUTDIR n_ch n_E [seed]\n");
   return 2;
 }

@@ -43,7 +43,23 @@ typedef enum ppm_status {
 /* Atmosphere model of a spot. */
 enum {
   PPM_ATM_BLACKBODY = 0, /* Ibb(E, kT), cpu/unit.hpp:28-31, os1 path            */
-  PPM_ATM_NSX = 1        /* NSX grid, 2l2c_le interpolation, cpu/nsx.hpp:248-296 */
+  PPM_ATM_NSX = 1        /* NSX grid, interpolant ppm_star.nsx_interp, cpu/nsx.hpp */
+};
+
+/* Interpolant of the NSX grid (ppm_star.nsx_interp).  Every axis look-up
+ * clamps to the axis ends (cpu/hunt.hpp:155-178); a cubic stencil starts one
+ * node below the bracket, clamped to [0, n-4].  t_interp's interp_type
+ * 0, 1, 2, 3 (cpu/t_interp.cpp:210-219) are 4C, 4C_SG, 4C_LE, 2L2C_LE. */
+enum {
+  PPM_NSX_2L2C_LE = 0, /* linear logT, log g; cubic logEkT; cubic mu, linear
+                          in the last mu brackets (Interp_IT3_2l2c_le,
+                          cpu/nsx.hpp:248-296; os1/devarshi programs)        */
+  PPM_NSX_4C = 1,      /* cubic in all four (Interp_IT3_4c, use_sg false,
+                          cpu/nsx.hpp:170-203)                                */
+  PPM_NSX_4C_SG = 2,   /* 4C with negative results set to 0 (use_sg true)   */
+  PPM_NSX_4C_LE = 3,   /* cubic logT, log g, logEkT; mu as 2L2C_LE
+                          (Interp_IT3_4c_le, cpu/nsx.hpp:205-246)             */
+  PPM_NSX_4L = 4       /* linear in all four (Interp_IT3_4l, cpu/nsx.hpp:298-321) */
 };
 
 /* Beaming of the blackbody atmosphere (cpu/os1_cpu.cpp:128-136). */
@@ -94,7 +110,7 @@ typedef struct ppm_star {
   int atmosphere;          /* PPM_ATM_*                                 */
   int variant;             /* PPM_VARIANT_*                             */
   int n_spots;             /* 1..PPM_MAX_SPOTS; contributions add       */
-  int reserved;
+  int nsx_interp;          /* PPM_NSX_* (NSX atmosphere; 0 = 2L2C_LE)   */
   ppm_spot spot[PPM_MAX_SPOTS];
 } ppm_star;
 
@@ -124,7 +140,9 @@ typedef struct ppm_open_opts {
   const char* nsx_path;          /* NULL -> ext/model_data/nsx_H_v200804.out;
                                     "" -> no NSX grid                                */
   const char* response_settings; /* comma list, tables/ism_inst/<s>; NULL/"" -> none */
-  const ppm_nsx_config* nsx_config; /* NULL -> nsx_H_v200804 (35x14x166x67)        */
+  const ppm_nsx_config* nsx_config; /* NULL -> nsx_H_v200804 (35x14x166x67);
+                                       nsx_H_v171019 is 35x11x166x67, log g
+                                       13.7-14.7 (cpu/nsx.hpp:39-51)         */
   int device;                    /* HIP device ordinal                               */
   int reserved;
   size_t hbm_budget_bytes;       /* serving admission budget; 0 -> 16 GiB            */
@@ -168,7 +186,9 @@ ppm_status ppm_lensing_domain(const ppm_ctx* ctx, double* u_min, double* u_max,
 /* Evaluate n profiles on one grid.  flux is [n][n_phase][n_energy] (FP64);
  * counts, if non-NULL and grid->response >= 0, is [n][n_ch][n_phase].
  * Host buffers; blocking.  Replaces calc_one_star
- * (gpu/os1_gpu.cu:303-349, gpu/devarshi24st_gpu.cu:431-511). */
+ * (gpu/os1_gpu.cu:303-349, gpu/devarshi24st_gpu.cu:431-511, and
+ * cpu/t_interp.cpp:19-224 with ppm_star.nsx_interp as its interp_type).
+ * In one call, stars may use different atmospheres and interpolants. */
 ppm_status ppm_eval(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid,
                     double* flux, double* counts, ppm_stats* stats);
 

@@ -412,49 +412,76 @@ static void lagrange4(double x, double x0, double x1, double x2, double x3, doub
 
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-/* NSX::Interp_IT3_2l2c_le (cpu/nsx.hpp:248-296): bilinear in (logT, logg),
- * 4-point Lagrange in logEkT (evaluated at the clamped logEkT), 4-point
- * Lagrange in mu, or linear when the mu bracket is among the last four. */
-static double nsx_interp(const or_nsx* t, double logT, double logg, double logEkT, double mu,
+/* Weights of one table axis: the 4-point Lagrange stencil starting one node
+ * below the bracket (clamped to [0, n-4]) at the clamped coordinate, or the
+ * bracket's two linear weights.  Returns the first node. */
+static int axis_weights(double x_min, double x_max, int n, double x, int cubic, double* w, int* nw,
+                        int64_t* n_clamped) {
+  bracket b = uniform_hunt(x_min, x_max, n, &x, n_clamped);
+  if (!cubic) { /* (j a + (1 - j)(1 - a)), cpu/nsx.hpp:284-289 */
+    *nw = 2;
+    w[0] = 1 - b.a;
+    w[1] = b.a;
+    return b.i;
+  }
+  int i = clampi(b.i - 1, 0, n - 4);
+  lagrange4(x, uniform_node(x_min, x_max, n, i), uniform_node(x_min, x_max, n, i + 1),
+            uniform_node(x_min, x_max, n, i + 2), uniform_node(x_min, x_max, n, i + 3), w);
+  *nw = 4;
+  return i;
+}
+
+/* The NSX interpolants (mode = PPM_NSX_*), each a weighted sum over a
+ * tensor stencil of the grid, accumulated in the reference's loop order
+ * (logT, log g, logEkT, mu) with weight wT wg wE wmu:
+ *   2L2C_LE  NSX::Interp_IT3_2l2c_le (cpu/nsx.hpp:248-296): linear logT and
+ *            log g, cubic logEkT, mu cubic or linear in the last brackets
+ *            (bracket index >= n_mu - 5);
+ *   4C, 4C_SG  NSX::Interp_IT3_4c (cpu/nsx.hpp:170-203): cubic in all four,
+ *            use_sg sets a negative result to 0;
+ *   4C_LE    NSX::Interp_IT3_4c_le (cpu/nsx.hpp:205-246): cubic logT, log g,
+ *            logEkT, mu as 2L2C_LE;
+ *   4L       NSX::Interp_IT3_4l (cpu/nsx.hpp:298-321): linear in all four.
+ * Axis coordinates are clamped to the axis ends by the hunt
+ * (cpu/hunt.hpp:155-178); the mu bracket extrapolates (VecHunt). */
+static double nsx_interp(const or_nsx* t, int mode, double logT, double logg, double logEkT, double mu,
                          int64_t* n_clamped) {
   const ppm_nsx_config* c = &t->cfg;
-  bracket bT = uniform_hunt(c->logT_min, c->logT_max, c->n_logT, &logT, n_clamped);
-  bracket bg = uniform_hunt(c->logg_min, c->logg_max, c->n_logg, &logg, n_clamped);
-  bracket bE = uniform_hunt(c->logEkT_min, c->logEkT_max, c->n_logEkT, &logEkT, n_clamped);
+  const int cubic_tg = mode == PPM_NSX_4C || mode == PPM_NSX_4C_SG || mode == PPM_NSX_4C_LE;
+  const int cubic_e = mode != PPM_NSX_4L;
+  double wT[4], wg[4], wE[4], wm[4];
+  int nT, ng, nE, nm;
+  int iT = axis_weights(c->logT_min, c->logT_max, c->n_logT, logT, cubic_tg, wT, &nT, n_clamped);
+  int ig = axis_weights(c->logg_min, c->logg_max, c->n_logg, logg, cubic_tg, wg, &ng, n_clamped);
+  int iE = axis_weights(c->logEkT_min, c->logEkT_max, c->n_logEkT, logEkT, cubic_e, wE, &nE, n_clamped);
   double a_mu;
   int i_mu = or_vec_hunt(t->mu, c->n_mu, mu, &a_mu);
-  int iE = clampi(bE.i - 1, 0, c->n_logEkT - 4);
-  double wE[4], wm[4];
-  lagrange4(logEkT, uniform_node(c->logEkT_min, c->logEkT_max, c->n_logEkT, iE),
-            uniform_node(c->logEkT_min, c->logEkT_max, c->n_logEkT, iE + 1),
-            uniform_node(c->logEkT_min, c->logEkT_max, c->n_logEkT, iE + 2),
-            uniform_node(c->logEkT_min, c->logEkT_max, c->n_logEkT, iE + 3), wE);
-  int jm_max;
-  if (i_mu >= c->n_mu - 5) {
-    jm_max = 1;
+  int linear_mu = mode == PPM_NSX_4L ||
+                  ((mode == PPM_NSX_2L2C_LE || mode == PPM_NSX_4C_LE) && i_mu >= c->n_mu - 5);
+  if (linear_mu) {
+    nm = 2;
     wm[0] = 1 - a_mu;
     wm[1] = a_mu;
   } else {
-    jm_max = 3;
+    nm = 4;
     i_mu = clampi(i_mu - 1, 0, c->n_mu - 4);
     lagrange4(mu, t->mu[i_mu], t->mu[i_mu + 1], t->mu[i_mu + 2], t->mu[i_mu + 3], wm);
   }
   long sT = (long)c->n_logg * c->n_logEkT * c->n_mu, sg = (long)c->n_logEkT * c->n_mu, sE = c->n_mu;
   double ret = 0;
-  for (int jT = 0; jT <= 1; ++jT)
-    for (int jg = 0; jg <= 1; ++jg)
-      for (int jE = 0; jE <= 3; ++jE)
-        for (int jm = 0; jm <= jm_max; ++jm) {
-          double v = t->data[(bT.i + jT) * sT + (bg.i + jg) * sg + (iE + jE) * sE + i_mu + jm];
-          double w = (jT * bT.a + (1 - jT) * (1 - bT.a)) * (jg * bg.a + (1 - jg) * (1 - bg.a)) *
-                     wE[jE] * wm[jm];
-          ret += v * w;
+  for (int jT = 0; jT < nT; ++jT)
+    for (int jg = 0; jg < ng; ++jg)
+      for (int jE = 0; jE < nE; ++jE)
+        for (int jm = 0; jm < nm; ++jm) {
+          double v = t->data[(iT + jT) * sT + (ig + jg) * sg + (iE + jE) * sE + i_mu + jm];
+          ret += v * (wT[jT] * wg[jg] * wE[jE] * wm[jm]);
         }
+  if (mode == PPM_NSX_4C_SG && ret < 0) ret = 0;
   return ret;
 }
 
-double or_nsx_interp(const or_nsx* nsx, double logT, double logg, double logEkT, double mu) {
-  return nsx_interp(nsx, logT, logg, logEkT, mu, NULL);
+double or_nsx_interp(const or_nsx* nsx, int mode, double logT, double logg, double logEkT, double mu) {
+  return nsx_interp(nsx, mode, logT, logg, logEkT, mu, NULL);
 }
 
 /* ----------------------------------------------------------- profile */
@@ -642,7 +669,7 @@ static void do_ring(job_t* J, const ring_t* ring, double* fl, double* zf, double
         for (int e = 0; e < ne; ++e) {
           double E = J->energies[e], Ee = E * zz;
           double logEkT = log10(Ee / kT_nsx);
-          double IT3 = nsx_interp(J->nsx, logT, logg, logEkT, mu, &J->n_clamped);
+          double IT3 = nsx_interp(J->nsx, st->nsx_interp, logT, logg, logEkT, mu, &J->n_clamped);
           double I = IT3 * T3 / PLANCK_ERG_S;
           out[e] += fI * I / E * w;
         }
@@ -746,6 +773,18 @@ int or_profile(const or_lensing* lt, const or_nsx* nsx, const or_response* rsp,
   if (star->atmosphere == PPM_ATM_NSX && !nsx) {
     snprintf(err, errlen, "NSX atmosphere requested without a grid");
     return 1;
+  }
+  if (star->atmosphere == PPM_ATM_NSX) {
+    const int m = star->nsx_interp;
+    if (m < PPM_NSX_2L2C_LE || m > PPM_NSX_4L) {
+      snprintf(err, errlen, "unknown NSX interpolant");
+      return 1;
+    }
+    if ((m == PPM_NSX_4C || m == PPM_NSX_4C_SG || m == PPM_NSX_4C_LE) &&
+        (nsx->cfg.n_logT < 4 || nsx->cfg.n_logg < 4)) {
+      snprintf(err, errlen, "cubic logT / log g stencils need 4 nodes per axis");
+      return 1;
+    }
   }
   int ne = grid->n_energy;
   if (!energies) {

@@ -75,7 +75,8 @@ void or_fold(const or_response* rsp, const double* flux, int n_phase, double sca
 /* Building blocks exposed for unit tests. */
 void or_healpix_ring(int n_side, int i, double* z, int* n_pix, double* s_shift);
 double or_healpix_phi(int n_side, int i, int j);
-double or_nsx_interp(const or_nsx* nsx, double logT, double logg, double logEkT, double mu);
+/* NSX interpolant mode = PPM_NSX_* at one point. */
+double or_nsx_interp(const or_nsx* nsx, int mode, double logT, double logg, double logEkT, double mu);
 void or_lensing_lookup(const or_lensing* lt, double u, double cos_psi, double* cos_alpha,
                        double* lf, double* cdt_over_R_at_ca);
 double or_gl_node(int n, int i, double a, double b);

@@ -59,7 +59,7 @@ def lib():
         L.or_healpix_phi.restype = c_double
         L.or_healpix_phi.argtypes = [c_int, c_int, c_int]
         L.or_nsx_interp.restype = c_double
-        L.or_nsx_interp.argtypes = [c_void_p, c_double, c_double, c_double, c_double]
+        L.or_nsx_interp.argtypes = [c_void_p, c_int, c_double, c_double, c_double, c_double]
         L.or_lensing_lookup.argtypes = [c_void_p, c_double, c_double, POINTER(c_double),
                                         POINTER(c_double), POINTER(c_double)]
         L.or_gl_node.restype = c_double

@@ -2,7 +2,8 @@
 inputs (tables/lensing/std, tables/ism_inst/std, ext/model_data/
 nsx_H_v200804.out), write the reference's output files, and their contents
 match the reference CPU programs' outputs on the same inputs
-(tests/golden/ref_cpu, made by cpu/os1_cpu.cpp and cpu/devarshi24st_cpu.cpp)
+(tests/golden/ref_cpu, made by cpu/os1_cpu.cpp, cpu/devarshi24st_cpu.cpp and
+cpu/t_interp.cpp)
 to 1e-4 max relative error per bin above 1e-6 of the peak."""
 import gzip
 import os
@@ -51,3 +52,23 @@ def test_devarshi24st_gpu(golden_root):
         assert got.shape == ref.shape == (270, 32)
         assert rel_err(got, ref) <= TOL, (name, setting, rel_err(got, ref))
         assert float(open(stem + "_time_gpu.txt").read()) > 0
+
+
+def test_t_interp_gpu(golden_root):
+    """The interpolant study (cpu/t_interp.cpp) as a drop-in: eight
+    output/test*_flux.txt files, compared at the sampled rows and energies
+    the reference's outputs are kept at (tests/golden/ref_cpu)."""
+    r = run("t_interp_gpu", golden_root)
+    assert r.stdout.count(">>> Total flux written to") == 8
+    for name in ("test10", "test11", "test12", "test13", "test20", "test21", "test22", "test23"):
+        path = os.path.join(GOLD, "t_interp_%s_sub.txt.gz" % name)
+        with gzip.open(path, "rt") as f:
+            rows = [int(v) for v in f.readline().split()[2:]]
+            cols = [int(v) for v in f.readline().split()[2:]]
+        ref = np.loadtxt(path)
+        got = np.loadtxt(os.path.join(golden_root, "output", "%s_flux.txt" % name))
+        assert got.shape == (256, 1024)
+        got = got[np.ix_(rows, cols)]
+        assert rel_err(got, ref) <= TOL, (name, rel_err(got, ref))
+        if name in ("test11", "test21"):  # use_sg
+            assert (got >= 0).all()

@@ -11,6 +11,7 @@ import os
 import numpy as np
 import pytest
 
+from gpu_ppm_amd import abi
 from gpu_ppm_amd import scenarios as S
 from oracle_binding import Oracle, rel_err
 
@@ -71,3 +72,54 @@ def test_planted_error_is_caught(oracle):
     flux, _ = oracle.profile(st, S.os1_grid(), threads=8)
     ref = np.loadtxt(os.path.join(GOLD, "os1f_cpu.txt"))
     assert rel_err(flux[:, 0], ref) > 1e-5
+
+
+# ------------------------------------------------------------- t_interp
+def _tinterp_ref(name):
+    path = os.path.join(GOLD, "t_interp_%s_sub.txt.gz" % name)
+    import gzip
+    with gzip.open(path, "rt") as f:
+        rows = [int(v) for v in f.readline().split()[2:]]
+        cols = [int(v) for v in f.readline().split()[2:]]
+    return np.loadtxt(path), rows, cols
+
+
+@pytest.fixture(scope="module")
+def tinterp_flux(oracle):
+    """Oracle flux of every t_interp star at the sampled energies."""
+    _, _, cols = _tinterp_ref("test10")
+    en = S.tinterp_energies()[cols]
+    out = {}
+    for name in S.TINTERP:
+        flux, _ = oracle.profile(S.tinterp_star(name), S.tinterp_grid(energies=en), threads=8)
+        out[name] = flux
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(S.TINTERP))
+def test_tinterp_interpolants(tinterp_flux, name):
+    """cpu/t_interp.cpp: one element at logT 6 through Interp_IT3_4c,
+    4c with use_sg, 4c_le and 2l2c_le (interp_type 0..3), 256 bins x 2 GL
+    nodes, 1024 fine phases; sampled rows and energies of its output."""
+    ref, rows, _ = _tinterp_ref(name)
+    flux = tinterp_flux[name][rows]
+    assert ref.shape == flux.shape == (33, 65)
+    assert rel_err(flux, ref) < PRINT_TOL
+    # below the comparison floor: the sign of the cubic's Wien-tail
+    # undershoot, and use_sg's zeros, agree entry by entry
+    mode = S.TINTERP_MODE[S.TINTERP[name][2]]
+    if mode == abi.NSX_4C_SG:
+        assert (ref >= 0).all() and (flux[ref == 0] == 0).all() and (flux >= 0).all()
+    else:
+        assert (flux[ref < 0] < 0).all() and (ref < 0).any()
+
+
+def test_tinterp_wrong_interpolant_is_caught(oracle):
+    """Each golden pins its own interpolant: evaluating test10 (4c) with
+    2l2c_le or 4l, or test12 (4c_le) with 4l, misses the reference by far
+    more than the print precision."""
+    ref, rows, cols = _tinterp_ref("test10")
+    en = S.tinterp_energies()[cols]
+    for mode in (abi.NSX_2L2C_LE, abi.NSX_4L):
+        flux, _ = oracle.profile(S.tinterp_star("test10", nsx_interp=mode), S.tinterp_grid(energies=en), threads=8)
+        assert rel_err(flux[rows], ref) > 1e-6

@@ -155,7 +155,7 @@ def test_nsx_interp_reproduces_cubics(poly_nsx):
         g = rng.uniform(cfg.logg_min, cfg.logg_max)
         e = rng.uniform(cfg.logEkT_min, cfg.logEkT_max)
         m = rng.uniform(mu_nodes[6], mu_nodes[0])  # bracket index < n_mu - 5: cubic branch
-        v = lib().or_nsx_interp(h, t, g, e, m)
+        v = lib().or_nsx_interp(h, abi.NSX_2L2C_LE, t, g, e, m)
         assert v == pytest.approx(_f(t, g, e, m, q), rel=1e-11)
 
 
@@ -168,14 +168,165 @@ def test_nsx_interp_linear_edge_and_clamp(poly_nsx):
     for _ in range(100):
         e = rng.uniform(cfg.logEkT_min, cfg.logEkT_max)
         m = rng.uniform(mu_nodes[-1], mu_nodes[-4])  # last brackets: linear in mu
-        assert lib().or_nsx_interp(hl, t, g, e, m) == pytest.approx(_f(t, g, e, m, ql), rel=1e-11)
+        assert lib().or_nsx_interp(hl, abi.NSX_2L2C_LE, t, g, e, m) == pytest.approx(_f(t, g, e, m, ql), rel=1e-11)
     # the linear edge is really linear: a cubic in mu is not reproduced there
     m = 0.5 * (mu_nodes[-2] + mu_nodes[-1])
-    assert abs(lib().or_nsx_interp(hc, t, g, 0.3, m) / _f(t, g, 0.3, m, qc) - 1) > 1e-6
+    assert abs(lib().or_nsx_interp(hc, abi.NSX_2L2C_LE, t, g, 0.3, m) / _f(t, g, 0.3, m, qc) - 1) > 1e-6
     # logEkT outside the axis is clamped to its end (cpu/hunt.hpp:160-172)
     m = 0.9
-    assert lib().or_nsx_interp(hc, t, g, 2.7, m) == pytest.approx(_f(t, g, cfg.logEkT_max, m, qc), rel=1e-11)
-    assert lib().or_nsx_interp(hc, t, g, -3.0, m) == pytest.approx(_f(t, g, cfg.logEkT_min, m, qc), rel=1e-11)
+    assert lib().or_nsx_interp(hc, abi.NSX_2L2C_LE, t, g, 2.7, m) == pytest.approx(_f(t, g, cfg.logEkT_max, m, qc), rel=1e-11)
+    assert lib().or_nsx_interp(hc, abi.NSX_2L2C_LE, t, g, -3.0, m) == pytest.approx(_f(t, g, cfg.logEkT_min, m, qc), rel=1e-11)
+
+
+# ------------------------------------------------- the other NSX interpolants
+def _write_grid(path, cfg, mu_nodes, fn):
+    """NSX-format text grid of fn(logT, logg, logEkT, mu) > 0."""
+    T = cfg.logT_min + (cfg.logT_max - cfg.logT_min) * np.arange(cfg.n_logT) / (cfg.n_logT - 1)
+    G = cfg.logg_min + (cfg.logg_max - cfg.logg_min) * np.arange(cfg.n_logg) / (cfg.n_logg - 1)
+    E = cfg.logEkT_min + (cfg.logEkT_max - cfg.logEkT_min) * np.arange(cfg.n_logEkT) / (cfg.n_logEkT - 1)
+    t, g, e, m = np.meshgrid(T, G, E, mu_nodes, indexing="ij")
+    v = fn(t, g, e, m)
+    assert (v > 0).all()
+    rows = np.stack([e.ravel(), m.ravel(), np.log10(v.ravel()), t.ravel(), g.ravel()], axis=1)
+    np.savetxt(path, rows, fmt="%.17g")
+
+
+CFG4 = dict(n_logT=6, n_logg=6, n_logEkT=20, n_mu=12)
+MU4 = np.cos(np.linspace(0.0, 1.5, 12))
+
+
+def _cub(a, b, c, d):
+    return lambda x: a + b * x + c * x * x + d * x ** 3
+
+
+PT, PG, PE, PM = _cub(3, 0.2, -0.1, 0.02), _cub(2, -0.3, 0.05, 0.01), _cub(4, 1, 0.3, 0.1), _cub(1, 1, 0.5, 0.2)
+LT, LG, LE, LM = _cub(3, 0.2, 0, 0), _cub(2, -0.1, 0, 0), _cub(4, 1, 0, 0), _cub(1, 2, 0, 0)
+
+
+@pytest.fixture(scope="module")
+def grids4(tmp_path_factory):
+    """Grids with >= 4 nodes on every axis: tensor products of cubics, of
+    lines, a cubic table with a linear mu factor, one that depends on logT
+    alone (sin), one on mu alone, and a steep Wien-like fall in logEkT."""
+    d = tmp_path_factory.mktemp("nsx4")
+    cfg = abi.nsx_config(**CFG4)
+    fns = {
+        "cubic": lambda t, g, e, m: PT(t) * PG(g) * PE(e) * PM(m),
+        "linear": lambda t, g, e, m: LT(t) * LG(g) * LE(e) * LM(m),
+        "cubic_lin_mu": lambda t, g, e, m: PT(t) * PG(g) * PE(e) * LM(m),
+        "sinT": lambda t, g, e, m: 2 + np.sin(4 * t) + 0 * (g + e + m),
+        "cosmu": lambda t, g, e, m: 2 + np.cos(5 * m) + 0 * (t + g + e),
+        "steep": lambda t, g, e, m: np.exp(-30 * e) * (1 + 0 * (t + g + m)),
+    }
+    out = {}
+    for name, fn in fns.items():
+        p = str(d / ("%s.out" % name))
+        _write_grid(p, cfg, MU4, fn)
+        err = ctypes.create_string_buffer(256)
+        h = lib().or_nsx_load(p.encode(), ctypes.byref(cfg), err, 256)
+        assert h, err.value
+        out[name] = (h, fn)
+    yield cfg, out
+    for h, _ in out.values():
+        lib().or_nsx_free(h)
+
+
+def _pts(cfg, rng, n, mu_lo=MU4[-1], mu_hi=MU4[0]):
+    return zip(rng.uniform(cfg.logT_min, cfg.logT_max, n), rng.uniform(cfg.logg_min, cfg.logg_max, n),
+               rng.uniform(cfg.logEkT_min, cfg.logEkT_max, n), rng.uniform(mu_lo, mu_hi, n))
+
+
+def test_nsx_4c_reproduces_tricubic(grids4):
+    """Interp_IT3_4c (cpu/nsx.hpp:170-203): cubic Lagrange in all four axes,
+    exact for a product of cubics at every in-range point, the last mu
+    brackets included; use_sg changes nothing where the value is positive."""
+    cfg, g = grids4
+    h, fn = g["cubic"]
+    for t, gg, e, m in _pts(cfg, np.random.default_rng(11), 400):
+        v = lib().or_nsx_interp(h, abi.NSX_4C, t, gg, e, m)
+        assert v == pytest.approx(fn(t, gg, e, m), rel=1e-11)
+        assert lib().or_nsx_interp(h, abi.NSX_4C_SG, t, gg, e, m) == v
+    # the bilinear (logT, log g) interpolants are not exact for it
+    assert abs(lib().or_nsx_interp(h, abi.NSX_2L2C_LE, 5.5, 14.0, 0.3, 0.9) / fn(5.5, 14.0, 0.3, 0.9) - 1) > 1e-6
+
+
+def test_nsx_4c_le_mu_edge(grids4):
+    """Interp_IT3_4c_le (cpu/nsx.hpp:205-246): cubic (logT, log g, logEkT);
+    in mu cubic away from the edge (equal to 4c there) and linear in the last
+    brackets (exact for a linear mu factor, not for a cubic one)."""
+    cfg, g = grids4
+    hc, fc = g["cubic"]
+    hl, fl = g["cubic_lin_mu"]
+    rng = np.random.default_rng(12)
+    for t, gg, e, m in _pts(cfg, rng, 200, MU4[6], MU4[0]):  # bracket < n_mu - 5
+        assert lib().or_nsx_interp(hc, abi.NSX_4C_LE, t, gg, e, m) == pytest.approx(fc(t, gg, e, m), rel=1e-11)
+    for t, gg, e, m in _pts(cfg, rng, 200, MU4[-1], MU4[-5]):
+        assert lib().or_nsx_interp(hl, abi.NSX_4C_LE, t, gg, e, m) == pytest.approx(fl(t, gg, e, m), rel=1e-11)
+    m = 0.5 * (MU4[-2] + MU4[-1])
+    assert abs(lib().or_nsx_interp(hc, abi.NSX_4C_LE, 6.0, 14.2, 0.3, m) / fc(6.0, 14.2, 0.3, m) - 1) > 1e-6
+    assert lib().or_nsx_interp(hc, abi.NSX_4C, 6.0, 14.2, 0.3, m) == pytest.approx(fc(6.0, 14.2, 0.3, m), rel=1e-11)
+
+
+def test_nsx_4l_multilinear_and_clamps(grids4):
+    """Interp_IT3_4l (cpu/nsx.hpp:298-321): linear in all four, exact for a
+    product of lines; coordinates beyond the uniform axes clamp to the ends
+    (cpu/hunt.hpp:160-172); not exact for cubics."""
+    cfg, g = grids4
+    h, fn = g["linear"]
+    for t, gg, e, m in _pts(cfg, np.random.default_rng(13), 400):
+        assert lib().or_nsx_interp(h, abi.NSX_4L, t, gg, e, m) == pytest.approx(fn(t, gg, e, m), rel=1e-12)
+    for t, tc in ((4.0, cfg.logT_min), (7.5, cfg.logT_max)):
+        for e, ec in ((-5.0, cfg.logEkT_min), (3.0, cfg.logEkT_max)):
+            assert lib().or_nsx_interp(h, abi.NSX_4L, t, 16.0, e, 0.7) == pytest.approx(fn(tc, cfg.logg_max, ec, 0.7),
+                                                                                       rel=1e-12)
+            hc, fc = g["cubic"]
+            assert lib().or_nsx_interp(hc, abi.NSX_4C, t, 12.0, e, 0.7) == pytest.approx(
+                fc(tc, cfg.logg_min, ec, 0.7), rel=1e-11)
+    hc, fc = g["cubic"]
+    assert abs(lib().or_nsx_interp(hc, abi.NSX_4L, 5.5, 14.0, 0.3, 0.9) / fc(5.5, 14.0, 0.3, 0.9) - 1) > 1e-4
+
+
+def _centred_cubic(nodes, vals, x):
+    """Textbook 4-point interpolation: the two nodes on each side of x,
+    shifted inward at the ends of the axis, through numpy's polynomial fit."""
+    nodes = np.asarray(nodes)
+    order = 1 if nodes[-1] > nodes[0] else -1
+    k = int(np.searchsorted(nodes[::order], x)) if order == 1 else len(nodes) - int(np.searchsorted(nodes[::-1], x))
+    lo = min(max(k - 2, 0), len(nodes) - 4)
+    c = np.polyfit(nodes[lo:lo + 4], vals[lo:lo + 4], 3)
+    return np.polyval(c, x)
+
+
+def test_nsx_cubic_stencils_are_centred(grids4):
+    """For functions no cubic reproduces (sin of logT, cos of mu) the 4c
+    stencil is the centred one: two nodes each side, shifted inward at the
+    axis ends (cpu/nsx.hpp:178-186)."""
+    cfg, g = grids4
+    hT, fT = g["sinT"]
+    nodes = cfg.logT_min + (cfg.logT_max - cfg.logT_min) * np.arange(cfg.n_logT) / (cfg.n_logT - 1)
+    for t in np.linspace(cfg.logT_min + 1e-3, cfg.logT_max - 1e-3, 57):
+        ref = _centred_cubic(nodes, fT(nodes, 0, 0, 0), t)
+        assert lib().or_nsx_interp(hT, abi.NSX_4C, t, 14.0, 0.0, 0.5) == pytest.approx(ref, rel=1e-9)
+    hm, fm = g["cosmu"]
+    for m in np.linspace(MU4[-1] + 1e-4, MU4[0] - 1e-4, 57):
+        ref = _centred_cubic(MU4, fm(0, 0, 0, MU4), m)
+        assert lib().or_nsx_interp(hm, abi.NSX_4C, 6.0, 14.0, 0.0, m) == pytest.approx(ref, rel=1e-9)
+
+
+def test_nsx_use_sg_clips_undershoot(grids4):
+    """A steep fall in logEkT (e^-30 per unit, ~200x per node) makes the cubic
+    undershoot below 0 between nodes; use_sg (cpu/nsx.hpp:201) returns
+    max(0, 4c) and 4c_le / 2l2c_le keep the negative value."""
+    cfg, g = grids4
+    h, _ = g["steep"]
+    n_neg = 0
+    for e in np.linspace(cfg.logEkT_min, cfg.logEkT_max, 301):
+        v = lib().or_nsx_interp(h, abi.NSX_4C, 6.0, 14.0, e, 0.5)
+        assert lib().or_nsx_interp(h, abi.NSX_4C_SG, 6.0, 14.0, e, 0.5) == max(v, 0.0)
+        assert lib().or_nsx_interp(h, abi.NSX_2L2C_LE, 6.0, 14.0, e, 0.5) == pytest.approx(v, rel=1e-12, abs=0)
+        assert lib().or_nsx_interp(h, abi.NSX_4L, 6.0, 14.0, e, 0.5) >= 0
+        n_neg += v < 0
+    assert n_neg > 50
 
 
 # ------------------------------------------------------------ lensing tables
