@@ -120,6 +120,29 @@ def test_extended_spots(ctx, orc, mode):
             assert rel_err(cp[0], cref) <= TOL
 
 
+@pytest.mark.parametrize("mode", MODES, ids=[NAMES[m] for m in MODES])
+def test_axis_clamps(ctx, orc, mode):
+    """Every axis beyond the table (cpu/hunt.hpp:160-172): logT below and
+    above (kT 0.005 and 1.2 keV: logT 4.8 and 7.1 against 5.1-6.8), log g
+    above (override 15.5 > 15.0), logEkT beyond both ends (0.01-100 keV).  A
+    wavefront's energies then span more slice rows than a row set holds, so
+    the per-energy evaluation paths run too.  Plain and batched."""
+    energies = S.geomspace(0.01, 100.0, 256)
+    for kT, logg in ((0.005, 0.0), (1.2, 15.5)):
+        st = with_mode(S.cfg1_star(), mode)
+        st.spot[0].kT_keV = kT
+        st.logg_override = logg
+        ref = None
+        for method in (abi.METHOD_PLAIN, abi.METHOD_BATCHED):
+            g = abi.make_grid(32, 16, sampling=abi.SAMPLE_GL, n_gl=2, energies=energies, method=method)
+            print("case: axis clamps", NAMES[mode], "kT", kT, "method", method, flush=True)
+            flux, _, _ = ctx.eval([st], g)
+            if ref is None:
+                ref, info = orc.profile(st, g, threads=THREADS)
+                assert info.n_clamped > 0
+            assert rel_err(flux[0], ref) <= TOL, (NAMES[mode], kT, method, rel_err(flux[0], ref))
+
+
 # ------------------------------------------------- a table with mu structure
 MU_NODES = np.cos(np.arange(67) / 66 * (0.5 * math.pi - 0.0175) * (0.35 + 0.65 * np.arange(67) / 66))
 
