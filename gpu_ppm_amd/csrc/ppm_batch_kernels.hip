@@ -401,9 +401,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 if (rows_path) {
                   if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next[0]);
                   if ((interior >> q[0]) & 1)
-                    nsx_eval_loaded<kBEPL, 1, false, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, val);
+                    nsx_eval_table<kBEPL, false, kBSets, EM>(le, pv, pr, q[0], ax, p_cur[0], lane, s_C, val[0]);
                   else
-                    nsx_eval_loaded<kBEPL, 1, true, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, val);
+                    nsx_eval_table<kBEPL, true, kBSets, EM>(le, pv, pr, q[0], ax, p_cur[0], lane, s_C, val[0]);
                 } else {
                   float w[4];
 #pragma unroll

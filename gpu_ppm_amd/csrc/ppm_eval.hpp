@@ -437,6 +437,65 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
     }
 }
 
+// The same evaluation for one patch with the coefficients passed through a
+// per-wavefront LDS table instead of cross-lane reads: per row set each lane
+// writes its row's four coefficients (one ds_write_b128) and each energy
+// reads its row's (one ds_read_b128), 2 + EPL LDS instructions per patch in
+// place of 4 EPL ds_bpermute.  The sets reuse the 64-entry table in turn:
+// a wavefront's LDS instructions complete in issue order, so the reads of
+// one set precede the next set's writes.
+template <int EPL, bool CLAMP, int SETS, int EM>
+__device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr, int q,
+                                      const EAxis& ax, const float (&p)[SETS][4], int lane, float4* s_T,
+                                      float (&out)[EPL]) {
+  constexpr int G = EPL / SETS;
+  constexpr bool kLin = EM == kELinear;
+  float w[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
+  const float tx = CLAMP ? readlane_f(pv.tx, q) : 0.f;
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
+    const float U = w[0] * p[s][0] + w[1] * p[s][1] + w[2] * p[s][2] + w[3] * p[s][3];
+    const float U1 = lane_next(U);
+    float4 a;
+    if (kLin) {
+      a = make_float4(U1, lane_next(U1) - U1, 0.f, 0.f);  // row f, row f+1 - row f
+    } else {
+      const float d1 = U1 - U;              // U(0) - U(-1)
+      const float D2 = lane_next(d1) - d1;  // second difference at x = 0
+      const float D3 = lane_next(D2) - D2;  // third difference
+      const float a2 = 0.5f * D2, a3 = (1.f / 6.f) * D3;
+      a = make_float4(U1, d1 + a2 - a3, a2, a3);
+    }
+    __builtin_amdgcn_wave_barrier();
+    s_T[lane] = a;
+    __builtin_amdgcn_wave_barrier();
+    const int i0 = CLAMP ? readlane_i(pr.i0[s], q) : 0;
+    const float txs = CLAMP ? 0.f : readlane_f(pr.txs[s], q);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int u = s * G + g;
+      int src;
+      float x;
+      if (CLAMP) {
+        const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
+        const int f = med3i(floor_to_int(t), ax.f_lo, ax.f_hi);
+        x = t - (float)f;
+        src = f - i0;
+      } else {
+        const float t = le[u] + txs;
+        src = floor_to_int(t);
+        x = __builtin_amdgcn_fractf(t);
+      }
+      const float4 c = s_T[src];
+      const float r = kLin ? fmaf(x, c.y, c.x) : fmaf(x, fmaf(x, fmaf(x, c.w, c.z), c.y), c.x);
+      out[u] = EM == kECubicClip ? fmaxf(r, 0.f) : r;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int EPL, int NP, bool CLAMP, int SETS, int EM>
 __device__ inline void nsx_eval_rows(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
                                      const int (&q)[NP], const EAxis& ax, const float* s_S, int lane,
