@@ -357,9 +357,10 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             }
           };
           auto fire = [&]() {
+            double* const row = acc + lane + ev_bin * n_e;  // the energies' offsets are immediates
 #pragma unroll
             for (int u = 0; u < kBEPL; ++u)
-              if (u * 64 + lane <= e_lim) unsafeAtomicAdd(acc + (size_t)ev_bin * n_e + u * 64 + lane, ev_sw * P[u]);
+              if (u * 64 + lane <= e_lim) unsafeAtomicAdd(row + u * 64, ev_sw * P[u]);
             ++ev;
             load_ev();
           };
