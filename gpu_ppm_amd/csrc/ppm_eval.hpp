@@ -441,9 +441,11 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
 // per-wavefront LDS table instead of cross-lane reads: per row set each lane
 // writes its row's four coefficients (one ds_write_b128) and each energy
 // reads its row's (one ds_read_b128), 2 + EPL LDS instructions per patch in
-// place of 4 EPL ds_bpermute.  The sets reuse the 64-entry table in turn:
-// a wavefront's LDS instructions complete in issue order, so the reads of
-// one set precede the next set's writes.
+// place of 4 EPL ds_bpermute.  Both sets' coefficients and every energy's
+// row and x are formed first, then the sets go through the 64-entry table in
+// turn with all EPL reads in flight before the first result is used: a
+// wavefront's LDS instructions complete in issue order, so the reads of one
+// set precede the next set's writes.
 template <int EPL, bool CLAMP, int SETS, int EM>
 __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr, int q,
                                       const EAxis& ax, const float (&p)[SETS][4], int lane, float4* s_T,
@@ -454,46 +456,57 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
 #pragma unroll
   for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
   const float tx = CLAMP ? readlane_f(pv.tx, q) : 0.f;
+  float4 a[SETS];
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
     const float U = w[0] * p[s][0] + w[1] * p[s][1] + w[2] * p[s][2] + w[3] * p[s][3];
     const float U1 = lane_next(U);
-    float4 a;
     if (kLin) {
-      a = make_float4(U1, lane_next(U1) - U1, 0.f, 0.f);  // row f, row f+1 - row f
+      a[s] = make_float4(U1, lane_next(U1) - U1, 0.f, 0.f);
     } else {
-      const float d1 = U1 - U;              // U(0) - U(-1)
-      const float D2 = lane_next(d1) - d1;  // second difference at x = 0
-      const float D3 = lane_next(D2) - D2;  // third difference
+      const float d1 = U1 - U;
+      const float D2 = lane_next(d1) - d1;
+      const float D3 = lane_next(D2) - D2;
       const float a2 = 0.5f * D2, a3 = (1.f / 6.f) * D3;
-      a = make_float4(U1, d1 + a2 - a3, a2, a3);
+      a[s] = make_float4(U1, d1 + a2 - a3, a2, a3);
     }
-    __builtin_amdgcn_wave_barrier();
-    s_T[lane] = a;
-    __builtin_amdgcn_wave_barrier();
+  }
+  int src[EPL];
+  float x[EPL];
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
     const int i0 = CLAMP ? readlane_i(pr.i0[s], q) : 0;
     const float txs = CLAMP ? 0.f : readlane_f(pr.txs[s], q);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int u = s * G + g;
-      int src;
-      float x;
       if (CLAMP) {
         const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
         const int f = med3i(floor_to_int(t), ax.f_lo, ax.f_hi);
-        x = t - (float)f;
-        src = f - i0;
+        x[u] = t - (float)f;
+        src[u] = f - i0;
       } else {
         const float t = le[u] + txs;
-        src = floor_to_int(t);
-        x = __builtin_amdgcn_fractf(t);
+        src[u] = floor_to_int(t);
+        x[u] = __builtin_amdgcn_fractf(t);
       }
-      const float4 c = s_T[src];
-      const float r = kLin ? fmaf(x, c.y, c.x) : fmaf(x, fmaf(x, fmaf(x, c.w, c.z), c.y), c.x);
-      out[u] = EM == kECubicClip ? fmaxf(r, 0.f) : r;
     }
   }
+  float4 c[EPL];
+#pragma unroll
+  for (int s = 0; s < SETS; ++s) {
+    __builtin_amdgcn_wave_barrier();
+    s_T[lane] = a[s];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int g = 0; g < G; ++g) c[s * G + g] = s_T[src[s * G + g]];
+  }
   __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int u = 0; u < EPL; ++u) {
+    const float r = kLin ? fmaf(x[u], c[u].y, c[u].x) : fmaf(x[u], fmaf(x[u], fmaf(x[u], c[u].w, c[u].z), c[u].y), c[u].x);
+    out[u] = EM == kECubicClip ? fmaxf(r, 0.f) : r;
+  }
 }
 
 template <int EPL, int NP, bool CLAMP, int SETS, int EM>
