@@ -181,13 +181,28 @@ void candidate_rings(const SpotParams& P, const ppm_spot& sp, int n_side, std::v
     return;
   }
   const double reach = sp.angrad1 + 1e-9;
-  for (int i = 1; i <= 4 * n_side - 1; ++i) {
+  const int n_ring = 4 * n_side - 1;
+  auto offset = [&](int i) {  // ring colatitude minus the spot centre's
     const bool north = i <= 2 * n_side;
     const int ii = north ? i : 4 * n_side - i;
     double z = ii < n_side ? 1. - 1. * ii * ii / (3. * n_side * n_side) : 4. / 3. - 2. * ii / (3. * n_side);
     if (!north) z = -z;
-    const double th = std::acos(std::max(-1.0, std::min(1.0, z)));
-    if (std::fabs(th - sp.theta1) <= reach) rings.push_back(i);
+    return std::acos(std::max(-1.0, std::min(1.0, z))) - sp.theta1;
+  };
+  // the colatitude grows with i, so the rings within reach are a contiguous
+  // range: bisect for its first ring, then test rings until one is beyond
+  int lo = 1, hi = n_ring + 1;
+  while (lo < hi) {
+    const int m = (lo + hi) / 2;
+    if (offset(m) >= -reach)
+      hi = m;
+    else
+      lo = m + 1;
+  }
+  for (int i = std::max(1, lo - 2); i <= n_ring; ++i) {
+    const double d = offset(i);
+    if (d > reach) break;
+    if (std::fabs(d) <= reach) rings.push_back(i);
   }
 }
 

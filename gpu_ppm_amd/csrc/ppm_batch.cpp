@@ -124,9 +124,11 @@ static BatchState* state(ppm_ctx* c) {
 
 // Fraction of a ring of colatitude th inside a disk (th1, rho), for cost
 // estimates only.
-static double arc_fraction(double th, double th1, double rho) {
-  const double s = std::sin(th) * std::sin(th1);
-  const double cth = (std::cos(rho) - std::cos(th) * std::cos(th1));
+// Fraction of a ring at z = cos(theta) within rho of a centre at th1, given
+// sin th1, cos th1 and cos rho.
+static double arc_fraction(double z, double st1, double ct1, double crho) {
+  const double s = std::sqrt(std::max(0.0, 1.0 - z * z)) * st1;
+  const double cth = crho - z * ct1;
   if (std::fabs(s) < 1e-15) return cth < 0 ? 1.0 : 0.0;
   const double x = cth / s;
   if (x <= -1) return 1.0;
@@ -336,6 +338,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
         spot_group.push_back(k);
         candidate_rings(P, G.stars[s].spot[q], g.n_side, rings);
         const ppm_spot& spt = G.stars[s].spot[q];
+        const double st1 = std::sin(spt.theta1), ct1 = std::cos(spt.theta1), crho = std::cos(spt.angrad1);
         for (int r : rings) {
           tasks.push_back(RingTask{si, r});
           double est;
@@ -344,7 +347,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
           } else {
             const int npix = 4 * std::min({r, 4 * g.n_side - r, g.n_side});
             const ClassSet cs = cls[k]->tab.sets[npix / 4];
-            const double a = arc_fraction(std::acos(ring_z(g.n_side, r)), spt.theta1, spt.angrad1) * npix;
+            const double a = arc_fraction(ring_z(g.n_side, r), st1, ct1, crho) * npix;
             const double per_cls = (double)G.pts.size() / std::max(cs.n_classes, 1);
             est = cs.n_classes * (std::min((double)npix, a * per_cls) + 64.0);
           }
@@ -463,6 +466,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       (e = BS->d_order.ensure(sizeof(int) * std::max(n_items, 1))) ||
       (e = BS->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) ||
       (e = W.evals.ensure(sizeof(unsigned long long))) ||
+      (e = W.summary.ensure(sizeof(unsigned long long) * (4 * n_groups + 1))) ||
       (e = BS->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
       (e = BS->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
@@ -512,6 +516,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
     Q.d_fine = (float*)W.fine.p + fine_off[k];
     Q.n_tasks = group_task0[k + 1] - group_task0[k];
     Q.n_fine = groups[k].g.n_fine;
+    Q.summary = (unsigned long long*)W.summary.p + 4 * k;
   }
   BatchLaunch B{};
   B.spots = (const SpotParams*)W.spots.p;
@@ -523,6 +528,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   B.star_item0 = (const int*)BS->d_star_item0.p;
   B.queue = (int*)W.counter.p;
   B.n_evals = (unsigned long long*)W.evals.p;
+  B.rows_flag = (unsigned long long*)W.summary.p + 4 * n_groups;
   B.acc = (double*)BS->d_acc.p;
   B.seg = (double*)BS->d_seg.p;
   B.n_items = n_items;
@@ -562,6 +568,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   auto record = [&](bool timed) -> int {
     int r;
     if (timed && (r = hipEventRecord(c->ev0, c->stream))) return r;
+    if ((r = launch_zero_u64((unsigned long long*)W.summary.p, 4 * n_groups + 1, c->stream))) return r;
     for (int k = 0; k < n_groups; ++k)
       if (P[k].n_tasks && (r = launch_ring_setup(c->L, c->N, P[k], c->stream))) return r;
     if (timed && (r = hipEventRecord(c->ev_i0, c->stream))) return r;
@@ -617,21 +624,18 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   }
   if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
 
-  std::vector<RingDesc> desc(n_tasks);
-  if (n_tasks && (e = hipMemcpy(desc.data(), W.desc.p, sizeof(RingDesc) * n_tasks, hipMemcpyDeviceToHost)))
-    return hip_fail(c, e, "descriptor read-back");
-  bool rows_overflow = false;
-  for (const RingDesc& d : desc) rows_overflow |= (d.flags & 2) != 0;
-  if (rows_overflow && nsx_rows < c->N.nE) return eval_groups_rows(c, groups, stats, c->N.nE);
+  // the ring set-up's counters per grid and the integral's row-overflow flag
+  std::vector<unsigned long long> sum(4 * n_groups + 1);
+  if ((e = hipMemcpy(sum.data(), W.summary.p, sizeof(unsigned long long) * sum.size(), hipMemcpyDeviceToHost)))
+    return hip_fail(c, e, "summary read-back");
+  if ((sum[4 * n_groups] & 2) && nsx_rows < c->N.nE) return eval_groups_rows(c, groups, stats, c->N.nE);
   int64_t n_act = 0, n_eval_ref = 0, n_rings = 0;
-  for (int k = 0; k < n_groups; ++k)
-    for (int t = group_task0[k]; t < group_task0[k + 1]; ++t) {
-      const RingDesc& d = desc[t];
-      if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
-      n_act += d.n_act;
-      if (d.n_act) ++n_rings;
-      if (d.kept) n_eval_ref += (int64_t)d.n_act * (int64_t)groups[k].pts.size() * groups[k].energies.size();
-    }
+  for (int k = 0; k < n_groups; ++k) {
+    if (sum[4 * k + 3] & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
+    n_act += (int64_t)sum[4 * k];
+    n_rings += (int64_t)sum[4 * k + 2];
+    n_eval_ref += (int64_t)sum[4 * k + 1] * (int64_t)groups[k].pts.size() * (int64_t)groups[k].energies.size();
+  }
   if (stats) {
     float t02 = 0;
     hipEventElapsedTime(&t02, c->ev0, c->ev2);

@@ -64,6 +64,7 @@ struct BatchLaunch {
   int nsx_rows, cov_words, ev_cap, atmosphere;
   int epl;                 // energies per lane: 6, or 2 when every grid has <= 128 energies
   int emode;               // NsxEnergyMode shared by the launch's stars
+  unsigned long long* rows_flag;  // bit 1 set when a ring needs more NSX rows than staged
 };
 
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
@@ -73,6 +74,7 @@ size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 int batch_energies(int epl);  // energies per item
 int batched_prepare(int atmosphere, int epl, int emode, size_t lds);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);
+int launch_zero_u64(unsigned long long* p, int n, void* stream);  // a kernel, not a memset (graph capture)
 // flux[s - s0][b][e] for the stars [s0, s1) of one grid (G, its host copy).
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream);
 int batched_occupancy(int atmosphere, int epl, int emode, size_t lds, int* wg_per_cu);

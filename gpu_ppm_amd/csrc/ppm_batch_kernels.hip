@@ -118,6 +118,11 @@ __host__ __device__ inline BLds batch_layout(int nf1, int nsx_rows, int cov_word
   return l;
 }
 
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *q = 0;
@@ -214,7 +219,10 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         const int r_hi = clampi((int)floor(t_max + 0.01) - 1, 0, N.nE - 4) + 3;
         n_rows = r_hi - r_lo + 1;
         if (n_rows > rows_alloc) {
-          if (tid == 0) atomicOr(&B.desc[task].flags, 2);
+          if (tid == 0) {
+            atomicOr(&B.desc[task].flags, 2);
+            atomicOr(B.rows_flag, 2ull);
+          }
           n_rows = rows_alloc;
         }
         ax = energy_axis<EM>(N.nE, r_lo, n_rows, rows_alloc);
@@ -563,6 +571,12 @@ int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream
   const hipError_t e =
       hipLaunchKernel(batched_fn(B.atmosphere, B.epl, B.emode), dim3(n_wg), dim3(kBThreads), args, lds, st);
   return e != hipSuccess ? e : hipGetLastError();
+}
+
+int launch_zero_u64(unsigned long long* p, int n, void* stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, n);
+  return hipGetLastError();
 }
 
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream) {

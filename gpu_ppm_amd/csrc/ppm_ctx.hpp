@@ -46,9 +46,9 @@ struct Workspace {
 
 // Buffers of the batched path (ppm_batch.cpp).
 struct BatchWorkspace {
-  DevBuf spots, tasks, desc, ph, fine, pts, energies, items, scale, counter, evals;
+  DevBuf spots, tasks, desc, ph, fine, pts, energies, items, scale, counter, evals, summary;
   void release_all() {
-    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &items, &scale, &counter, &evals})
+    for (DevBuf* b : {&spots, &tasks, &desc, &ph, &fine, &pts, &energies, &items, &scale, &counter, &evals, &summary})
       b->release();
   }
 };

@@ -124,6 +124,9 @@ struct PlainLaunch {
   int atmosphere;
   int emode;             // NsxEnergyMode of the star's interpolant
   int nsx_rows;          // NSX slice rows staged per workgroup
+  // ring set-up counters, or null: [0] active patches, [1] active patches of
+  // kept rings, [2] rings with active patches, [3] flags of all rings (OR)
+  unsigned long long* summary;
 };
 
 // Kernel launchers (ppm_kernels.hip).  Return a hipError_t as int.

@@ -263,6 +263,11 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
   if (tid == 0) {
     D->n_act = n_act;
     D->n_runs = n_runs < kMaxRuns ? n_runs : kMaxRuns;
+    if (P.summary) {
+      atomicAdd(&P.summary[0], (unsigned long long)n_act);
+      if (n_act) atomicAdd(&P.summary[2], 1ull);
+      if (n_runs > kMaxRuns) atomicOr(&P.summary[3], 1ull);
+    }
   }
   if (n_act == 0) {
     if (tid == 0) D->kept = 0;
@@ -430,6 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
     for (int a = 0; a < 4; ++a)
       for (int b = 0; b < 4; ++b) D->wtg[4 * a + b] = (float)(wT[a] * wg[b]);
     D->kept = 1;
+    if (P.summary) atomicAdd(&P.summary[1], (unsigned long long)n_act);
   }
 }
 
