@@ -383,6 +383,8 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             // skipped patches add nothing to the prefix: the events between
             // two queued positions fire with the same prefix
             uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
+            n_done += n_valid * (unsigned int)__builtin_popcountll(todo);  // every visible patch is evaluated
+            uint64_t due = __ballot(k >= ev_pos);
             PatchRows<kBSets> pr{};
             if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
             const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.interior);
@@ -429,15 +431,16 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 }
               }
               // in position order: window ends at or before pos use the
-              // prefix before F(pos)
-              const int pos = readlane_i(k, q[0]);
-              if (ev_pos <= pos) {
+              // prefix before F(pos); `due` marks the lanes whose position
+              // is at or after the next window end
+              if ((due >> q[0]) & 1) {
+                const int pos = readlane_i(k, q[0]);
                 fold();
                 while (ev_pos <= pos) fire();
+                due = __ballot(k >= ev_pos);
               }
 #pragma unroll
               for (int u = 0; u < kBEPL; ++u) pf[u] += val[0][u];
-              n_done += n_valid;
             };
             float p_a[1][kBSets][4], p_b[1][kBSets][4];
             if (rows_path && todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a[0]);
