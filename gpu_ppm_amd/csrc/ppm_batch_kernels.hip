@@ -400,54 +400,62 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 pf[u] = 0.f;
               }
             };
-            // one patch at a time; with the row-set path the slice rows of
-            // the next patch are read while the current one is evaluated
-            // (two row buffers, used in turn)
-            const bool rows_path = ATM == PPM_ATM_NSX && we.fast;
-            auto patch_step = [&](const float (&p_cur)[1][kBSets][4], float (&p_next)[1][kBSets][4]) {
-              const int q[1] = {__builtin_ctzll(todo)};
-              todo &= todo - 1;
-              float val[1][kBEPL];
-              if (ATM == PPM_ATM_NSX) {
-                if (rows_path) {
-                  if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next[0]);
-                  if ((interior >> q[0]) & 1)
-                    nsx_eval_table<kBEPL, false, kBSets, EM>(le, pv, pr, q[0], ax, p_cur[0], lane, s_C, val[0]);
-                  else
-                    nsx_eval_table<kBEPL, true, kBSets, EM>(le, pv, pr, q[0], ax, p_cur[0], lane, s_C, val[0]);
-                } else {
-                  float w[4];
-#pragma unroll
-                  for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[0]);
-                  nsx_eval_edge<kBEPL, EM>(le, asc, readlane_f(pv.tx, q[0]), readlane_i(pv.im, q[0]), w, ax, s_S,
-                                           s_C, lane, val[0]);
-                }
-              } else {
-                const float c = readlane_f(pv.w[0], q[0]), z = readlane_f(pv.zf, q[0]);
-#pragma unroll
-                for (int u = 0; u < kBEPL; ++u) {
-                  const float Ee = Ef[u] * z;
-                  val[0][u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
-                }
-              }
-              // in position order: window ends at or before pos use the
-              // prefix before F(pos); `due` marks the lanes whose position
-              // is at or after the next window end
-              if ((due >> q[0]) & 1) {
-                const int pos = readlane_i(k, q[0]);
+            // in position order: window ends at or before pos use the
+            // prefix before F(pos); `due` marks the lanes whose position is
+            // at or after the next window end
+            auto accumulate = [&](int q, const float (&val)[kBEPL]) {
+              if ((due >> q) & 1) {
+                const int pos = readlane_i(k, q);
                 fold();
                 while (ev_pos <= pos) fire();
                 due = __ballot(k >= ev_pos);
               }
 #pragma unroll
-              for (int u = 0; u < kBEPL; ++u) pf[u] += val[0][u];
+              for (int u = 0; u < kBEPL; ++u) pf[u] += val[u];
             };
-            float p_a[1][kBSets][4], p_b[1][kBSets][4];
-            if (rows_path && todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a[0]);
-            while (todo) {
-              patch_step(p_a, p_b);
-              if (!todo) break;
-              patch_step(p_b, p_a);
+            // one patch at a time.  With the row-set path the slice rows of
+            // the next patch are read while the current one is evaluated (two
+            // row buffers, used in turn); the path is chosen once per batch
+            if (ATM == PPM_ATM_NSX && we.fast) {
+              auto step_rows = [&](const float (&p_cur)[kBSets][4], float (&p_next)[kBSets][4]) {
+                const int q = __builtin_ctzll(todo);
+                todo &= todo - 1;
+                if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
+                float val[kBEPL];
+                if ((interior >> q) & 1)
+                  nsx_eval_table<kBEPL, false, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+                else
+                  nsx_eval_table<kBEPL, true, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+                accumulate(q, val);
+              };
+              float p_a[kBSets][4], p_b[kBSets][4];
+              if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a);
+              while (todo) {
+                step_rows(p_a, p_b);
+                if (!todo) break;
+                step_rows(p_b, p_a);
+              }
+            } else {
+              while (todo) {
+                const int q = __builtin_ctzll(todo);
+                todo &= todo - 1;
+                float val[kBEPL];
+                if (ATM == PPM_ATM_NSX) {
+                  float w[4];
+#pragma unroll
+                  for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
+                  nsx_eval_edge<kBEPL, EM>(le, asc, readlane_f(pv.tx, q), readlane_i(pv.im, q), w, ax, s_S, s_C,
+                                           lane, val);
+                } else {
+                  const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
+#pragma unroll
+                  for (int u = 0; u < kBEPL; ++u) {
+                    const float Ee = Ef[u] * z;
+                    val[u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
+                  }
+                }
+                accumulate(q, val);
+              }
             }
             fold();
             // drop the batch from the queue
