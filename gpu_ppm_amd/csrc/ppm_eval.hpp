@@ -473,18 +473,23 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
   }
   int src[EPL];
   float x[EPL];
+  // the clamps of the row f in the float domain (floor and med3 of a float,
+  // f exact), the set's first row folded into the table pointer
+  const float f_lo = (float)ax.f_lo, f_hi = (float)ax.f_hi;
+  const float4* T[SETS];
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
     const int i0 = CLAMP ? readlane_i(pr.i0[s], q) : 0;
     const float txs = CLAMP ? 0.f : readlane_f(pr.txs[s], q);
+    T[s] = s_T - i0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int u = s * G + g;
       if (CLAMP) {
         const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
-        const int f = med3i(floor_to_int(t), ax.f_lo, ax.f_hi);
-        x[u] = t - (float)f;
-        src[u] = f - i0;
+        const float f = __builtin_amdgcn_fmed3f(floorf(t), f_lo, f_hi);
+        x[u] = t - f;
+        src[u] = (int)f;
       } else {
         const float t = le[u] + txs;
         src[u] = floor_to_int(t);
@@ -499,7 +504,7 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
     s_T[lane] = a[s];
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int g = 0; g < G; ++g) c[s * G + g] = s_T[src[s * G + g]];
+    for (int g = 0; g < G; ++g) c[s * G + g] = T[s][src[s * G + g]];
   }
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
