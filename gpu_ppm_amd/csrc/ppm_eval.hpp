@@ -474,7 +474,9 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
   int src[EPL];
   float x[EPL];
   // the clamps of the row f in the float domain (floor and med3 of a float,
-  // f exact), the set's first row folded into the table pointer
+  // f exact), the set's first row folded into the table pointer (s_T - i0
+  // stays inside the dynamic LDS block: the table lies behind the ring
+  // tables and the slice, far more than 16 i0 <= 16 EAxis::i0_max bytes)
   const float f_lo = (float)ax.f_lo, f_hi = (float)ax.f_hi;
   const float4* T[SETS];
 #pragma unroll
