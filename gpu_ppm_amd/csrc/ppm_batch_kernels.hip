@@ -235,6 +235,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const int mu_mode = nsx_mu_mode(S.nsx_interp);
       const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
       const bool point = T.ring == 0;
+      const bool ring_hidden = Dp->hidden != 0;
       const int n = point ? 1 : Dp->n_pix;
       const double inv_n = 1.0 / n;
       const int n_runs = point ? 1 : Dp->n_runs;
@@ -307,8 +308,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           // ---- keep the covered positions whose patch is seen (flux/I != 0):
           // the others add nothing, so the split, the queue and the set-up
           // only count patches that contribute.  Each wavefront rewrites its
-          // own 64-position chunks.
-          for (int base = wave * 64; base < n; base += kBThreads) {
+          // own 64-position chunks.  A ring without invisible fine nodes has
+          // flux/I > 0 between all of them: nothing to drop.
+          for (int base = wave * 64; ring_hidden && base < n; base += kBThreads) {
             const int k = base + lane;
             bool keep = false;
             if (k < n && ((s_cov[k >> 5] >> (k & 31)) & 1u)) {

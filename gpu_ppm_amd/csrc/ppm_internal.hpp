@@ -74,7 +74,7 @@ struct RingDesc {
   int flags;                  // bit0: run overflow, bit1: NSX rows beyond the staged slice
   int iT, ig;                 // first (logT, log g) node of the NSX stencil
   int n_tg;                   // stencil nodes per axis: 2 (linear) or 4 (cubic)
-  int pad;
+  int hidden;                 // 1 if some fine node is invisible (flux/I = 0 there)
   float wtg[16];              // weight of node (iT + a, ig + b) at [4 a + b]
   int run_start[kMaxRuns];    // first active pixel j (1-based) of each run
   int run_len[kMaxRuns];

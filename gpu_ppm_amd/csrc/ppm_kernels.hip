@@ -375,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
     if (tid == 0) D->kept = 0;
     return;
   }
+  if (tid == 0) D->hidden = inv_min <= inv_max;
   if (inv_min <= inv_max) {  // gap fill (cpu/os1_cpu.cpp:152-163)
     const int beg = inv_min - 1, end = inv_max + 1;
     const double p0 = ph[beg], p1 = ph[end];
