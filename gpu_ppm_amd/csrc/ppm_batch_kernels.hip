@@ -389,7 +389,12 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             uint64_t due = __ballot(k >= ev_pos);
             PatchRows<kBSets> pr{};
             if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
-            const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.interior);
+            // patches evaluated without clamps, and those whose low row set
+            // takes the clamped evaluation (the rest clamp the high set only:
+            // energies beyond the end of the axis)
+            static_assert(kBSets == 2, "the clamp classes assume two row sets");
+            const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.clamp == 0);
+            const uint64_t clamp_lo = __ballot(lane < cnt && pv.im >= 0 && (pr.clamp & 1));
             // FP32 partial sums of at most one batch, folded into the FP64
             // prefix before an event fires and at the end of the batch
             float pf[kBEPL];
@@ -425,9 +430,11 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
                 float val[kBEPL];
                 if ((interior >> q) & 1)
-                  nsx_eval_table<kBEPL, false, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+                  nsx_eval_table<kBEPL, 0, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+                else if ((clamp_lo >> q) & 1)
+                  nsx_eval_table<kBEPL, 3, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
                 else
-                  nsx_eval_table<kBEPL, true, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+                  nsx_eval_table<kBEPL, 2, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
                 accumulate(q, val);
               };
               float p_a[kBSets][4], p_b[kBSets][4];

@@ -275,9 +275,10 @@ __device__ inline WaveEnergies<SETS> wave_energies(const float (&le)[EPL], int l
 
 // Classification of one patch, made once in its set-up lane.  When the
 // wavefront's energies allow nsx_eval_rows (WaveEnergies::fast) the patch is
-// evaluated there: without clamps when no energy reaches a clamp of the axis
-// or of the row f (checked on the first and last energy of each set, the
-// evaluation being monotone in the energy), with them otherwise.
+// evaluated there, each row set without clamps when none of its energies
+// reaches a clamp of the axis or of the row f (checked on the set's first and
+// last energy, the evaluation being monotone in the energy), with them
+// otherwise.
 // Set s holds slice rows i0[s] - 1 + lane: i0[s] is at most the row f of the
 // set's first energy, and at most EAxis::i0_max so that every lane's row is
 // in the staged allocation.  txs = tx - i0 puts a set's
@@ -289,13 +290,13 @@ struct PatchRows {
   int i0[SETS];
   float txs[SETS];
   int rb[SETS];
-  bool interior;  // no clamps
+  int clamp;  // bit s: set s takes the clamped evaluation (every bit without WaveEnergies::fast)
 };
 
 template <int SETS>
 __device__ inline PatchRows<SETS> patch_rows(float tx, int im, const WaveEnergies<SETS>& we, const EAxis& ax) {
   PatchRows<SETS> r;
-  r.interior = we.fast;
+  r.clamp = 0;
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
     const float tc = __builtin_amdgcn_fmed3f(we.le_first[s] + tx, ax.t_lo, ax.t_hi);
@@ -307,7 +308,7 @@ __device__ inline PatchRows<SETS> patch_rows(float tx, int im, const WaveEnergie
     // (the rounding of le + tx may differ near an integer)
     const float uf = we.le_first[s] + r.txs[s], ul = we.le_last[s] + r.txs[s];
     const float i0f = (float)r.i0[s];
-    r.interior = r.interior && uf >= 0.f && ul < (float)(ax.f_hi + 1) - i0f && ul <= ax.t_hi - i0f;
+    if (!(we.fast && uf >= 0.f && ul < (float)(ax.f_hi + 1) - i0f && ul <= ax.t_hi - i0f)) r.clamp |= 1 << s;
   }
   return r;
 }
@@ -326,7 +327,7 @@ __device__ inline int floor_to_int(float t) {
 
 // Contracted evaluation of NP patches at once (independent chains, for
 // latency hiding), every patch of a wavefront with WaveEnergies::fast and,
-// with CLAMP = false, PatchRows::interior; patch j is the one set up in lane
+// with CLAMP = false, no bit of PatchRows::clamp; patch j is the one set up in lane
 // q[j] (pv, pr).  For energy t = le + tx (clamped to the axis ends) the row
 // is f = floor(t) (clamped to EAxis::f_lo..f_hi) and x = t - f, in [0, 1)
 // away from the clamps.  For each row set, lane l contracts slice row
@@ -445,8 +446,9 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
 // row and x are formed first, then the sets go through the 64-entry table in
 // turn with all EPL reads in flight before the first result is used: a
 // wavefront's LDS instructions complete in issue order, so the reads of one
-// set precede the next set's writes.
-template <int EPL, bool CLAMP, int SETS, int EM>
+// set precede the next set's writes.  Bit s of CM selects the clamped
+// evaluation for set s (PatchRows::clamp).
+template <int EPL, int CM, int SETS, int EM>
 __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr, int q,
                                       const EAxis& ax, const float (&p)[SETS][4], int lane, float4* s_T,
                                       float (&out)[EPL]) {
@@ -455,7 +457,7 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
   float w[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
-  const float tx = CLAMP ? readlane_f(pv.tx, q) : 0.f;
+  const float tx = CM ? readlane_f(pv.tx, q) : 0.f;
   float4 a[SETS];
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
@@ -481,13 +483,14 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
   const float4* T[SETS];
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
-    const int i0 = CLAMP ? readlane_i(pr.i0[s], q) : 0;
-    const float txs = CLAMP ? 0.f : readlane_f(pr.txs[s], q);
+    const bool clamp = (CM >> s) & 1;
+    const int i0 = clamp ? readlane_i(pr.i0[s], q) : 0;
+    const float txs = clamp ? 0.f : readlane_f(pr.txs[s], q);
     T[s] = s_T - i0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int u = s * G + g;
-      if (CLAMP) {
+      if (clamp) {
         const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
         const float f = __builtin_amdgcn_fmed3f(floorf(t), f_lo, f_hi);
         x[u] = t - f;

@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
         uint64_t todo = __ballot(pv.im >= 0);
         PatchRows<1> pr{};
         if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
-        const uint64_t interior = __ballot(pv.im >= 0 && pr.interior);
+        const uint64_t interior = __ballot(pv.im >= 0 && pr.clamp == 0);
         while (todo) {
           int qs[kNP];  // a missing patch repeats the first
           bool all_interior = true;
