@@ -97,6 +97,13 @@ constexpr size_t kAccCapBytes = (size_t)12 << 30;
 // resident workgroups per CU.
 constexpr int kDefaultItemsPerSlot = 12;
 
+// The last kTailChunks chunks of each spot's rings (of at least
+// kTailChunks + 1) are cut kTailSplit times finer.  Items are taken
+// largest-first, so the queue ends with small items and the persistent
+// workgroups finish closer together.
+constexpr int kTailChunks = 3;
+constexpr int kTailSplit = 4;
+
 struct BatchState {
   std::map<std::vector<int>, ClassCache> classes;  // per (n_side, n_phase, sampling, n_gl)
   uint64_t use_clock = 0;
@@ -398,6 +405,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   std::vector<double> item_cost;
   std::vector<int> star_item0(n + 1, 0);
   size_t acc_total = 0;
+  std::vector<double> cuts;
   auto build_items = [&](double target) {
     items.clear();
     item_cost.clear();
@@ -412,11 +420,16 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
         for (int t = t0; t < t1; ++t) cs += cost[t];
         int k = (int)std::lround(total_cost > 0 ? target * cs / total_cost / n_echunk[k_grid] : 1.0);
         k = std::max(1, std::min(k, t1 - t0));
+        // cumulative cost at the end of each piece
+        const int kt = k > kTailChunks ? kTailChunks : 0;
+        cuts.clear();
+        for (int j = 1; j <= k - kt; ++j) cuts.push_back(cs * j / k);
+        for (int j = 1; j <= kt * kTailSplit; ++j) cuts.push_back(cs * (k - kt) / k + cs * j / ((double)k * kTailSplit));
         double acc = 0;
         int start = t0, made = 0;
         for (int t = t0; t < t1; ++t) {
           acc += cost[t];
-          if (t == t1 - 1 || acc >= cs * (made + 1) / k) {
+          if (t == t1 - 1 || (made < (int)cuts.size() && acc >= cuts[made])) {
             double ic = 0;
             for (int u = start; u <= t; ++u) ic += cost[u];
             for (int ec = 0; ec < n_echunk[k_grid]; ++ec) {
