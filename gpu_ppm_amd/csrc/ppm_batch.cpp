@@ -95,7 +95,7 @@ constexpr size_t kAccCapBytes = (size_t)12 << 30;
 // workgroup (how finely each spot's rings are split; more items balance the
 // persistent workgroups better, fewer carry less per-item set-up) and
 // resident workgroups per CU.
-constexpr int kDefaultItemsPerSlot = 12;
+constexpr int kDefaultItemsPerSlot = 10;
 
 // The last kTailChunks chunks of each spot's rings (of at least
 // kTailChunks + 1) are cut kTailSplit times finer.  Items are taken

@@ -33,7 +33,7 @@ def test_tune_keeps_the_fastest_shape_and_results_are_unchanged(ctx):
     r = ctx.tune(stars, g, repeats=1)
     assert r.n_tried >= 4
     assert 0 < r.best_ms <= r.default_ms
-    assert r.best.items_per_slot in (3, 6, 12, 24, 48) and r.best.wg_per_cu >= 1
+    assert r.best.items_per_slot in (2, 5, 10, 20, 40) and r.best.wg_per_cu >= 1  # default 10, halved or doubled
     assert ctx.launch_shape(g, len(stars), abi.ATM_NSX) == (r.best.items_per_slot, r.best.wg_per_cu)
     f1, c1, _ = ctx.eval(stars, g)  # tuned shape
     for k in range(len(stars)):
