@@ -5,15 +5,17 @@
 // A worker thread owned by the context takes, whenever the device is free,
 // every queued request whose launch class (atmosphere and, for NSX, the
 // interpolant's energy-axis evaluation) is the oldest request's (up to
-// kMaxBatch, in arrival order), groups them by grid and evaluates all of
-// them in one persistent batched launch (several grids per launch), while
-// new requests keep arriving.  Results are copied to host memory of the
-// request and handed out by ppm_poll in any order.  The reference has no
+// kMaxBatch, in arrival order), groups them by grid and evaluates them in
+// persistent batched launches (several grids per launch, the cheapest launch
+// first), while new requests keep arriving.  The results of a launch are
+// copied to host memory of its requests as soon as it ends and handed out by
+// ppm_poll in any order.  The reference has no
 // serving layer; each request is one calc_one_star
 // (gpu/devarshi24st_gpu.cu:431-511) without the text output.
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <numeric>
 #include <unordered_map>
 
 #include "ppm_batch.hpp"
@@ -116,6 +118,62 @@ void worker_loop(ppm_ctx* c) {
       c_off[gi + 1] = c_off[gi] + m * (size_t)r0.n_ch * r0.grid.n_phase;
     }
     std::vector<ppm_status> st(n_groups, PPM_OK);
+    // hands the results of groups gis (already in host memory, or failed)
+    // to their requests
+    auto publish = [&](const std::vector<int>& gis) {
+      const Clock::time_point now = Clock::now();
+      {
+        std::lock_guard<std::mutex> lk(S.mu);
+        for (int gi : gis) {
+          const size_t per_flux = (f_off[gi + 1] - f_off[gi]) / members[gi].size();
+          const size_t per_counts = (c_off[gi + 1] - c_off[gi]) / members[gi].size();
+          for (size_t j = 0; j < members[gi].size(); ++j) {
+            const int k = members[gi][j];
+            Request* r = batch[k];
+            S.charged -= r->bytes;
+            --S.active;
+            if (r->cancel_pending) {
+              S.reqs.erase(tickets[k]);
+              continue;
+            }
+            r->t_done = now;
+            r->status = st[gi];
+            if (st[gi] == PPM_OK) {
+              const double* f = (const double*)S.h_flux.p + f_off[gi] + per_flux * j;
+              r->flux.assign(f, f + per_flux);
+              if (per_counts) {
+                const double* q = (const double*)S.h_counts.p + c_off[gi] + per_counts * j;
+                r->counts.assign(q, q + per_counts);
+              }
+              r->state = ReqState::kDone;
+            } else {
+              r->state = ReqState::kFailed;
+            }
+          }
+        }
+      }
+      S.cv_done.notify_all();
+    };
+    // copies the results of groups gis to host memory, then publishes them
+    auto finish = [&](const std::vector<int>& gis) {
+      int e = 0;
+      for (int gi : gis) {
+        if (st[gi] != PPM_OK) continue;
+        if (!e && f_off[gi + 1] > f_off[gi])
+          e = hipMemcpyAsync((double*)S.h_flux.p + f_off[gi], (const double*)S.d_flux.p + f_off[gi],
+                             sizeof(double) * (f_off[gi + 1] - f_off[gi]), hipMemcpyDeviceToHost, c->stream);
+        if (!e && c_off[gi + 1] > c_off[gi])
+          e = hipMemcpyAsync((double*)S.h_counts.p + c_off[gi], (const double*)S.d_counts.p + c_off[gi],
+                             sizeof(double) * (c_off[gi + 1] - c_off[gi]), hipMemcpyDeviceToHost, c->stream);
+      }
+      if (!e) e = hipStreamSynchronize(c->stream);
+      if (e) {
+        const ppm_status q = hip_fail(c, e, "serving read-back");
+        for (int gi : gis)
+          if (st[gi] == PPM_OK) st[gi] = q;
+      }
+      publish(gis);
+    };
     {
       std::lock_guard<std::mutex> lk(c->exec_mu);
       hipSetDevice(c->device);
@@ -125,6 +183,9 @@ void worker_loop(ppm_ctx* c) {
           (c_off[n_groups] && ((e = S.d_counts.ensure(sizeof(double) * c_off[n_groups])) ||
                                (e = S.h_counts.ensure(sizeof(double) * c_off[n_groups]))))) {
         st.assign(n_groups, hip_fail(c, e, "serving buffers"));
+        std::vector<int> all(n_groups);
+        std::iota(all.begin(), all.end(), 0);
+        publish(all);
       } else {
         std::vector<BatchGroup> groups(n_groups);
         stars.assign(n_groups, {});
@@ -173,12 +234,25 @@ void worker_loop(ppm_ctx* c) {
           }
           if (!cur_idx.empty()) launches.push_back(cur_idx);
         }
+        // cheapest launch first, each launch's requests completed as soon as
+        // it ends (work ~ stars x pixels x phase points x energies)
+        auto work = [&](const std::vector<int>& L) {
+          double w = 0;
+          for (int gi : L) {
+            const BatchGroup& G = groups[gi];
+            w += (double)G.n * G.g.n_side * G.g.n_side * (double)G.pts.size() * (double)G.energies.size();
+          }
+          return w;
+        };
+        std::stable_sort(launches.begin(), launches.end(),
+                         [&](const std::vector<int>& a, const std::vector<int>& b) { return work(a) < work(b); });
         for (const std::vector<int>& L : launches) {
           std::vector<BatchGroup> lg;
           for (int gi : L) lg.push_back(groups[gi]);
           ppm_stats stats{};
           const ppm_status q = eval_batched_groups(c, lg, &stats);
           for (int gi : L) st[gi] = q;
+          finish(L);
         }
         for (int gi : plain) {
           ppm_grid g = groups[gi].g;
@@ -186,50 +260,10 @@ void worker_loop(ppm_ctx* c) {
           g.method = PPM_METHOD_PLAIN;
           ppm_stats stats{};
           st[gi] = eval_any(c, groups[gi].stars, groups[gi].n, &g, groups[gi].d_flux, groups[gi].d_counts, &stats);
-        }
-        if ((e = hipMemcpyAsync(S.h_flux.p, S.d_flux.p, sizeof(double) * f_off[n_groups], hipMemcpyDeviceToHost,
-                                c->stream)) ||
-            (c_off[n_groups] && (e = hipMemcpyAsync(S.h_counts.p, S.d_counts.p, sizeof(double) * c_off[n_groups],
-                                                    hipMemcpyDeviceToHost, c->stream))) ||
-            (e = hipStreamSynchronize(c->stream))) {
-          const ppm_status q = hip_fail(c, e, "serving read-back");
-          for (ppm_status& x : st)
-            if (x == PPM_OK) x = q;
+          finish({gi});
         }
       }
     }
-    const Clock::time_point now = Clock::now();
-    {
-      std::lock_guard<std::mutex> lk(S.mu);
-      for (int gi = 0; gi < n_groups; ++gi) {
-        const size_t per_flux = (f_off[gi + 1] - f_off[gi]) / members[gi].size();
-        const size_t per_counts = (c_off[gi + 1] - c_off[gi]) / members[gi].size();
-        for (size_t j = 0; j < members[gi].size(); ++j) {
-          const int k = members[gi][j];
-          Request* r = batch[k];
-          S.charged -= r->bytes;
-          --S.active;
-          if (r->cancel_pending) {
-            S.reqs.erase(tickets[k]);
-            continue;
-          }
-          r->t_done = now;
-          r->status = st[gi];
-          if (st[gi] == PPM_OK) {
-            const double* f = (const double*)S.h_flux.p + f_off[gi] + per_flux * j;
-            r->flux.assign(f, f + per_flux);
-            if (per_counts) {
-              const double* q = (const double*)S.h_counts.p + c_off[gi] + per_counts * j;
-              r->counts.assign(q, q + per_counts);
-            }
-            r->state = ReqState::kDone;
-          } else {
-            r->state = ReqState::kFailed;
-          }
-        }
-      }
-    }
-    S.cv_done.notify_all();
   }
 }
 
