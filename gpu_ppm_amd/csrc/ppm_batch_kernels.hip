@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             // prefix before F(pos); `due` marks the lanes whose position is
             // at or after the next window end
             auto accumulate = [&](int q, const float (&val)[kBEPL]) {
-              if ((due >> q) & 1) {
+              if (__builtin_expect((due >> q) & 1, 0)) {
                 const int pos = readlane_i(k, q);
                 fold();
                 while (ev_pos <= pos) fire();
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 todo &= todo - 1;
                 if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
                 float val[kBEPL];
-                if ((interior >> q) & 1)
+                if (__builtin_expect((interior >> q) & 1, 1))
                   nsx_eval_table<kBEPL, 0, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
                 else if ((clamp_lo >> q) & 1)
                   nsx_eval_table<kBEPL, 3, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
