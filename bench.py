@@ -85,6 +85,14 @@ def cpu_baseline(root, stars, lensing, target_s=20.0):
                       "%.1f s on %d threads; scaled by %d/%d" % (len(sample), n_e, N_E, t, threads, N_E, n_e)}
 
 
+def rel_err(x, ref, floor=1e-6):
+    """max |x - ref| / |ref| over the entries with |ref| > floor * max|ref|."""
+    import numpy as np
+    x, ref = np.asarray(x, dtype=np.float64), np.asarray(ref, dtype=np.float64)
+    m = np.abs(ref) > floor * np.abs(ref).max()
+    return float((np.abs(x[m] - ref[m]) / np.abs(ref[m])).max()) if m.any() else 0.0
+
+
 def timed(ctx, stars, g, d_flux, d_cnt, steps, warmup):
     from gpu_ppm_amd import hip
     for _ in range(warmup):
@@ -141,6 +149,17 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
     hip.synchronize()
     t_plain = time.perf_counter() - t0
     log("bench: plain %d stars in %.2f s" % (len(sub), t_plain))
+    # outside the timed regions: the batched results of the direct launch
+    # against the plain path on every 8th timed star
+    fb, cb = d_flux.numpy(), d_cnt.numpy()
+    checked = list(range(0, len(sub), 8))
+    err = {"flux": 0.0, "counts": 0.0, "stars_checked": len(checked)}
+    for i in checked:
+        ctx.eval_device([stars[i]], gp, d_f1.ptr, d_c1.ptr)
+        hip.synchronize()
+        err["flux"] = max(err["flux"], rel_err(fb[i], d_f1.numpy()[0]))
+        err["counts"] = max(err["counts"], rel_err(cb[i], d_c1.numpy()[0]))
+    log("bench: batched vs plain max rel err %s" % err)
     ctx.close()
     del os.environ["PPM_NO_GRAPH"]
     k = sk[0]
@@ -150,7 +169,7 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
                 plain_profiles_per_s=len(sub) / t_plain, plain_stars=len(sub),
                 plain_evals_per_s=ref_evals / (plain_int_ms * 1e-3),
                 graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms,
-                launch_shape=shape)
+                launch_shape=shape, max_rel_err=err)
 
 
 def bench_serving(root, duration_s, burst, seed=7):
@@ -329,6 +348,7 @@ def main():
         out["plain"] = {"profiles_per_s": r["plain_profiles_per_s"], "stars_timed": r["plain_stars"],
                         "evals_per_s_ref": r["plain_evals_per_s"]}
         out["batched_vs_plain"] = r["value"] / r["plain_profiles_per_s"]
+        out["batched_vs_plain_max_rel_err"] = r["max_rel_err"]
         out["batched"] = {"graph_ms_per_step": r["graph_ms"], "integral_ms": r["kernel_ms"],
                           "setup_reduce_fold_ms": r["setup_reduce_fold_ms"], "n_eval_ref": r["n_eval_ref"],
                           "dedupe_factor": r["n_eval_ref"] / max(r["n_eval"], 1),
