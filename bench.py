@@ -181,7 +181,9 @@ def bench_serving(root, duration_s, burst, seed=7):
     energies (folded through the 300 x 128 or 300 x 360 response).  The
     capacity is first measured on a burst of `burst` requests submitted at
     once; the arrival rate is then 80 % of it for `duration_s` seconds.
-    Latency is submit -> result on the host, reported by ppm_poll."""
+    Latency is submit -> result on the host, reported by ppm_poll.  Every
+    100th request of the timed stream is audited afterwards against a direct
+    plain-path evaluation of the same star and grid."""
     import numpy as np
     from gpu_ppm_amd import abi
     from gpu_ppm_amd import scenarios as S
@@ -198,23 +200,25 @@ def bench_serving(root, duration_s, burst, seed=7):
     def draw(n):
         return [(walk.next(), combos[rng.integers(len(combos))]) for _ in range(n)]
 
-    def run(reqs, arrivals):
+    def run(reqs, arrivals, audit=None):
         pending, lat, i = {}, [], 0
         t0 = time.perf_counter()
         while i < len(reqs) or pending:
             now = time.perf_counter() - t0
             while i < len(reqs) and (arrivals is None or arrivals[i] <= now):
                 star, c = reqs[i]
-                pending[ctx.submit(star, grids[c])] = c
+                pending[ctx.submit(star, grids[c])] = (c, i)
                 i += 1
             for t in list(pending):
-                c = pending[t]
-                st, _, _, l = ctx.poll(t, grids[c], out=bufs[c])
+                c, k = pending[t]
+                st, f, q, l = ctx.poll(t, grids[c], out=bufs[c])
                 if st == abi.PENDING:
                     continue
                 if st != abi.OK:
                     raise RuntimeError("serving request failed: status %d" % st)
                 lat.append(l)
+                if audit is not None and k % 100 == 0:
+                    audit[k] = (np.array(f, copy=True), np.array(q, copy=True))
                 del pending[t]
             time.sleep(0.0005)
         return time.perf_counter() - t0, np.array(lat)
@@ -227,7 +231,16 @@ def bench_serving(root, duration_s, burst, seed=7):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     reqs = draw(n_req)
     log("bench: serving %d requests at %.1f/s (capacity %.1f/s)" % (n_req, rate, capacity))
-    dt, lat = run(reqs, arrivals)
+    audit = {}
+    dt, lat = run(reqs, arrivals, audit)
+    err = {"flux": 0.0, "counts": 0.0, "requests_audited": len(audit)}
+    for k, (f, q) in audit.items():
+        star, c = reqs[k]
+        g = abi.make_grid(c[0], c[1], sampling=abi.SAMPLE_GL, n_gl=2, response=grids[c].response,
+                          method=abi.METHOD_PLAIN)
+        f0, q0, _ = ctx.eval([star], g)
+        err["flux"] = max(err["flux"], rel_err(f, f0[0]))
+        err["counts"] = max(err["counts"], rel_err(q, q0[0]))
     ref_evals = 0
     for star, c in reqs:
         ref_evals += ctx.count_active(star, grids[c]) * c[1] * 2 * c[2]
@@ -235,7 +248,7 @@ def bench_serving(root, duration_s, burst, seed=7):
     return {"requests": n_req, "offered_per_s": rate, "burst_capacity_per_s": capacity,
             "completed_per_s": n_req / dt, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
-            "evals_per_s_ref": ref_evals / dt,
+            "evals_per_s_ref": ref_evals / dt, "audit_max_rel_err_vs_plain": err,
             "workload": "prior random walk (2%% steps), N_side {256,1024,2048} x bins {32,64,128} x 2 GL "
                         "x E {128,360}, open-loop Poisson arrivals at 80%% of burst capacity for %.0f s"
                         % duration_s}
