@@ -391,12 +391,11 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   // ---- items: each spot's rings split into chunks of similar cost, times
   // the energy chunks of its grid; about per_slot items per resident
   // workgroup, fewer if their accumulators would pass kAccCapBytes
+  // an item's accumulator copies cover its own energy chunk only
   std::vector<int> n_echunk(n_groups);
-  std::vector<size_t> acc_per_item(n_groups);
   for (int k = 0; k < n_groups; ++k) {
     const int ne = (int)groups[k].energies.size();
     n_echunk[k] = (ne + per_item - 1) / per_item;
-    acc_per_item[k] = (size_t)groups[k].g.n_phase * ne * batch_acc_copies();
   }
   double total_cost = 0;
   for (double v : cost) total_cost += v;
@@ -432,9 +431,11 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
           if (t == t1 - 1 || (made < (int)cuts.size() && acc >= cuts[made])) {
             double ic = 0;
             for (int u = start; u <= t; ++u) ic += cost[u];
+            const int ne = (int)groups[k_grid].energies.size();
             for (int ec = 0; ec < n_echunk[k_grid]; ++ec) {
-              items.push_back(BatchItem{start, t + 1, s, si, ec * per_item, k_grid, acc_total});
-              acc_total += acc_per_item[k_grid];
+              const int e0 = ec * per_item, en = std::min(per_item, ne - e0);
+              items.push_back(BatchItem{start, t + 1, s, si, e0, k_grid, en, acc_total});
+              acc_total += (size_t)groups[k_grid].g.n_phase * en * batch_acc_copies();
               item_cost.push_back(ic);
             }
             start = t + 1;

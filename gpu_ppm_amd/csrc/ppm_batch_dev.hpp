@@ -44,7 +44,8 @@ struct BatchItem {
   int star, spot;
   int e0;      // first energy of the chunk
   int grid;    // index into BatchLaunch::grids
-  size_t acc;  // first double of the item's accumulator copies, [copies][n_phase][n_e]
+  int en;      // energies in the chunk (its accumulator row length)
+  size_t acc;  // first double of the item's accumulator copies, [copies][n_phase][en]
 };
 
 struct BatchLaunch {

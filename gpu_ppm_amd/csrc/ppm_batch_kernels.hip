@@ -170,8 +170,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
     const int n_e = G.n_e, n_phase = G.n_phase, nf1 = G.n_fine + 1;
     const double* energies = G.energies;
     // one accumulator copy per position part: each address has one writer;
-    // acc points at the item's first energy
-    double* acc = B.acc + It.acc + (size_t)wp * n_phase * n_e + It.e0;
+    // a copy holds the item's own energies, rows of It.en
+    const int en = It.en;
+    double* acc = B.acc + It.acc + (size_t)wp * n_phase * en;
     // energies of this thread: e = e0 + u*64 + lane (ascending grid), valid
     // while u*64 + lane <= e_lim
     const int e_hi = min(It.e0 + kBEnergies, n_e) - 1, e_lim = e_hi - It.e0;
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
     for (int b = 0; b < n_phase; ++b)
 #pragma unroll
       for (int u = 0; u < kBEPL; ++u)
-        if (u * 64 + lane <= e_lim) acc[(size_t)b * n_e + u * 64 + lane] = 0.0;
+        if (u * 64 + lane <= e_lim) acc[(size_t)b * en + u * 64 + lane] = 0.0;
 
     for (int task = It.task0; task < It.task1; ++task) {
       const RingDesc* Dp = B.desc + task;
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             }
           };
           auto fire = [&]() {
-            double* const row = acc + lane + ev_bin * n_e;  // the energies' offsets are immediates
+            double* const row = acc + lane + ev_bin * en;  // the energies' offsets are immediates
 #pragma unroll
             for (int u = 0; u < kBEPL; ++u)
               if (u * 64 + lane <= e_lim) unsafeAtomicAdd(row + u * 64, ev_sw * P[u]);
@@ -529,9 +530,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   if (lane == 0) atomicAdd(B.n_evals, tot);
 }
 
-// flux[s - s0][b][e] = sum over the star's items (in item order) of
-// scale(spot) * (sum of the item's accumulator copies) / E, for the stars
-// [s0, s1) of one grid.
+// flux[s - s0][b][e] = sum over the star's items whose energy chunk holds e
+// (in item order) of scale(spot) * (sum of the item's accumulator copies) /
+// E, for the stars [s0, s1) of one grid.
 __global__ void k_reduce_batched(BatchLaunch B, int s0, int s1, int n_phase, int n_e, const double* energies,
                                  double* flux) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -539,14 +540,18 @@ __global__ void k_reduce_batched(BatchLaunch B, int s0, int s1, int n_phase, int
   if (idx >= per * (s1 - s0)) return;
   const int s = s0 + (int)(idx / per);
   const size_t o = idx - (size_t)(s - s0) * per;
+  const int b = (int)(o / n_e), e = (int)(o - (size_t)b * n_e);
   double total = 0.0;
   for (int it = B.star_item0[s]; it < B.star_item0[s + 1]; ++it) {
     const BatchItem I = B.items[it];
+    if (e < I.e0 || e >= I.e0 + I.en) continue;  // another chunk's item: it wrote none of e
+    const size_t cp = (size_t)n_phase * I.en;
+    const size_t oi = I.acc + (size_t)b * I.en + (e - I.e0);
     double a = 0.0;
-    for (int c = 0; c < kBWP; ++c) a += B.acc[I.acc + c * per + o];
+    for (int c = 0; c < kBWP; ++c) a += B.acc[oi + c * cp];
     total += B.spots[I.spot].scale * a;
   }
-  flux[idx] = total / energies[o % n_e];
+  flux[idx] = total / energies[e];
 }
 
 }  // namespace

@@ -246,7 +246,7 @@ typedef struct ppm_fit {
 ppm_status ppm_eval_fit(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid, const double* data,
                         int data_per_star, ppm_fit* fit, double* counts, ppm_stats* stats);
 
-/* Launch shape of the batched path.  A 0 field means the default: 12
+/* Launch shape of the batched path.  A 0 field means the default: 10
  * work-queue items per resident workgroup (each spot's rings are cut into
  * chunks of similar cost so that the queue holds about that many items per
  * workgroup), and the occupancy limit of workgroups per CU. */

@@ -122,3 +122,23 @@ def test_batched_equals_plain_full_resolution(ctx):
         assert rel_err(cb[k], cp[k]) <= 1e-5, (k, rel_err(cb[k], cp[k]))
     assert sb.n_active == sp.n_active and sb.n_eval_ref == sp.n_eval_ref
     assert 0 < sb.n_eval_done < sb.n_eval_ref
+
+
+def test_energy_chunks_after_a_larger_batch(ctx, orc):
+    """Items of one energy chunk hold accumulators for their own energies
+    only: a batch with more than 384 energies, run right after a larger,
+    different batch on the same context (whose accumulator memory it reuses),
+    still matches the oracle in every chunk."""
+    first = S.sample_priors(12, seed=31)
+    print("case: larger batch first (12 stars, N 64, 360 E)", flush=True)
+    ctx.eval(first, batch_grid(64, 32, response=0))
+    energies = S.geomspace(0.2, 12.0, 500)
+    g = abi.make_grid(48, 16, sampling=abi.SAMPLE_GL, n_gl=2, energies=energies, method=abi.METHOD_BATCHED)
+    stars = S.sample_priors(3, seed=32)
+    print("case: then 3 stars, N 48, 500 E (two energy chunks)", flush=True)
+    flux, _, _ = ctx.eval(stars, g)
+    for k, st in enumerate(stars):
+        ref, _ = orc.profile(st, g, threads=16)
+        e = rel_err(flux[k], ref)
+        print("  star %d: %.2e (high chunk %.2e)" % (k, e, rel_err(flux[k][:, 384:], ref[:, 384:])), flush=True)
+        assert e <= TOL, (k, e)
