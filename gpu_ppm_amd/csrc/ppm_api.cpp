@@ -300,17 +300,29 @@ int nsx_rows_for(const ppm_ctx* c, const std::vector<double>& energies) {
 
 ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                            const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
-                           double* d_scale_slot, ppm_stats* stats, int rows_override);
+                           double* d_scale_slot, ppm_stats* stats, int rows_override, bool no_alloc);
 
 ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                       const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
                       double* d_scale_slot, ppm_stats* stats) {
-  return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, 0);
+  return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, 0, false);
 }
+
+// Device buffers of one plain evaluation, carved in order from one region
+// (the no-allocation scratch) with 256-byte alignment.
+struct Carve {
+  unsigned char* base;
+  size_t cap, used = 0;
+  void* take(size_t bytes) {
+    const size_t o = (used + 255) & ~(size_t)255;
+    used = o + std::max<size_t>(bytes, 1);
+    return used <= cap ? base + o : nullptr;
+  }
+};
 
 ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                            const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
-                           double* d_scale_slot, ppm_stats* stats, int rows_override) {
+                           double* d_scale_slot, ppm_stats* stats, int rows_override, bool no_alloc) {
   const int n_e = (int)energies.size();
   const int nf1 = g.n_fine + 1;
   std::vector<SpotParams> sp(st.n_spots);
@@ -326,33 +338,71 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   if (nsx_rows) nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
   if (integral_lds_bytes(g.n_fine, st.atmosphere, nsx_rows) > (size_t)c->max_lds)
     return fail(c, PPM_ERR_ARG, "n_fine too large for the LDS of one workgroup");
-  Workspace& W = c->ws;
   int e = 0;
-  const size_t n_part = (size_t)std::max(n_tasks, 1) * g.n_phase * n_e;
-  if ((e = W.spots.ensure(sizeof(SpotParams) * sp.size())) || (e = W.tasks.ensure(sizeof(RingTask) * std::max(n_tasks, 1))) ||
-      (e = W.desc.ensure(sizeof(RingDesc) * std::max(n_tasks, 1))) ||
-      (e = W.ph.ensure(sizeof(double) * nf1 * std::max(n_tasks, 1))) ||
-      (e = W.fine.ensure(sizeof(float) * 3 * nf1 * std::max(n_tasks, 1))) ||
-      (e = W.pts.ensure(sizeof(PhasePoint) * pts.size())) || (e = W.energies.ensure(sizeof(double) * n_e)) ||
-      (e = W.partial.ensure(sizeof(float) * n_part)) ||
-      (e = W.red.ensure(sizeof(double) * reduce_groups(n_tasks) * g.n_phase * n_e)))
-    return hip_fail(c, e, "device allocation");
-  Upload up;
-  up.add(W.spots.p, sp.data(), sizeof(SpotParams) * sp.size());
-  if (n_tasks) up.add(W.tasks.p, tasks.data(), sizeof(RingTask) * n_tasks);
-  up.add(W.pts.p, pts.data(), sizeof(PhasePoint) * pts.size());
-  up.add(W.energies.p, energies.data(), sizeof(double) * n_e);
-  if ((e = up.run(c))) return hip_fail(c, e, "upload");
+  const size_t nt1 = (size_t)std::max(n_tasks, 1);
+  const size_t b_spots = sizeof(SpotParams) * sp.size(), b_tasks = sizeof(RingTask) * nt1,
+               b_desc = sizeof(RingDesc) * nt1, b_ph = sizeof(double) * nf1 * nt1,
+               b_fine = sizeof(float) * 3 * nf1 * nt1, b_pts = sizeof(PhasePoint) * pts.size(),
+               b_en = sizeof(double) * n_e, b_part = sizeof(float) * nt1 * g.n_phase * n_e,
+               b_red = sizeof(double) * reduce_groups(n_tasks) * g.n_phase * n_e;
+  void *p_spots, *p_tasks, *p_desc, *p_ph, *p_fine, *p_pts, *p_en, *p_part, *p_red;
+  if (no_alloc) {
+    Carve cv{(unsigned char*)c->noalloc, kNoAllocScratchBytes};
+    p_spots = cv.take(b_spots);
+    p_tasks = cv.take(b_tasks);
+    p_desc = cv.take(b_desc);
+    p_ph = cv.take(b_ph);
+    p_fine = cv.take(b_fine);
+    p_pts = cv.take(b_pts);
+    p_en = cv.take(b_en);
+    p_part = cv.take(b_part);
+    p_red = cv.take(b_red);
+    d_scale_slot = (double*)cv.take(sizeof(double));
+    if (!c->noalloc || cv.used > cv.cap)
+      return fail(c, PPM_ERR_NOMEM,
+                  "this profile needs " + std::to_string(cv.used) + " bytes of device scratch; the no-allocation "
+                  "entry point has " + std::to_string(kNoAllocScratchBytes) + " (use ppm_eval_device)");
+    // host arrays copied directly (no pinned staging buffer, which would
+    // have to be allocated); they stay alive until the stream is
+    // synchronised below
+    if (((e = hipMemcpyAsync(p_spots, sp.data(), b_spots, hipMemcpyHostToDevice, c->stream))) ||
+        (n_tasks && (e = hipMemcpyAsync(p_tasks, tasks.data(), sizeof(RingTask) * n_tasks, hipMemcpyHostToDevice,
+                                        c->stream))) ||
+        (e = hipMemcpyAsync(p_pts, pts.data(), b_pts, hipMemcpyHostToDevice, c->stream)) ||
+        (e = hipMemcpyAsync(p_en, energies.data(), b_en, hipMemcpyHostToDevice, c->stream)))
+      return hip_fail(c, e, "upload");
+  } else {
+    Workspace& W = c->ws;
+    if ((e = W.spots.ensure(b_spots)) || (e = W.tasks.ensure(b_tasks)) || (e = W.desc.ensure(b_desc)) ||
+        (e = W.ph.ensure(b_ph)) || (e = W.fine.ensure(b_fine)) || (e = W.pts.ensure(b_pts)) ||
+        (e = W.energies.ensure(b_en)) || (e = W.partial.ensure(b_part)) || (e = W.red.ensure(b_red)))
+      return hip_fail(c, e, "device allocation");
+    p_spots = W.spots.p;
+    p_tasks = W.tasks.p;
+    p_desc = W.desc.p;
+    p_ph = W.ph.p;
+    p_fine = W.fine.p;
+    p_pts = W.pts.p;
+    p_en = W.energies.p;
+    p_part = W.partial.p;
+    p_red = W.red.p;
+    Upload up;
+    up.add(p_spots, sp.data(), b_spots);
+    if (n_tasks) up.add(p_tasks, tasks.data(), sizeof(RingTask) * n_tasks);
+    up.add(p_pts, pts.data(), b_pts);
+    up.add(p_en, energies.data(), b_en);
+    if ((e = up.run(c))) return hip_fail(c, e, "upload");
+  }
 
   PlainLaunch P{};
-  P.d_spots = (const SpotParams*)W.spots.p;
-  P.d_tasks = (const RingTask*)W.tasks.p;
-  P.d_desc = (RingDesc*)W.desc.p;
-  P.d_ph = (double*)W.ph.p;
-  P.d_fine = (float*)W.fine.p;
-  P.d_pts = (const PhasePoint*)W.pts.p;
-  P.d_energies = (const double*)W.energies.p;
-  P.d_partial = (float*)W.partial.p;
+  P.d_spots = (const SpotParams*)p_spots;
+  P.d_tasks = (const RingTask*)p_tasks;
+  P.d_desc = (RingDesc*)p_desc;
+  P.d_ph = (double*)p_ph;
+  P.d_fine = (float*)p_fine;
+  P.d_pts = (const PhasePoint*)p_pts;
+  P.d_energies = (const double*)p_en;
+  P.d_partial = (float*)p_part;
   P.d_flux = d_flux;
   P.n_tasks = n_tasks;
   P.n_fine = g.n_fine;
@@ -372,7 +422,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   if (n_tasks == 0) {
     if ((e = hipMemsetAsync(d_flux, 0, sizeof(double) * g.n_phase * n_e, c->stream)))
       return hip_fail(c, e, "clear");
-  } else if ((e = launch_reduce(P, (double*)W.red.p, c->stream))) {
+  } else if ((e = launch_reduce(P, (double*)p_red, c->stream))) {
     return hip_fail(c, e, "reduce launch");
   }
   if ((e = hipEventRecord(c->ev1, c->stream))) return hip_fail(c, e, "event");
@@ -389,7 +439,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   if ((e = hipEventSynchronize(c->ev2))) return hip_fail(c, e, "execution");
   // read back the descriptors for statistics and run-overflow detection
   std::vector<RingDesc> desc(n_tasks);
-  if (n_tasks && (e = hipMemcpy(desc.data(), W.desc.p, sizeof(RingDesc) * n_tasks, hipMemcpyDeviceToHost)))
+  if (n_tasks && (e = hipMemcpy(desc.data(), p_desc, sizeof(RingDesc) * n_tasks, hipMemcpyDeviceToHost)))
     return hip_fail(c, e, "descriptor read-back");
   int64_t n_act = 0, n_eval = 0, n_rings = 0;
   bool rows_overflow = false;
@@ -397,7 +447,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   if (rows_overflow && nsx_rows < c->N.nE) {
     // a ring's redshift range needs more NSX rows than were staged: redo
     // with the whole slice (results of the first pass are discarded)
-    return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, c->N.nE);
+    return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, c->N.nE, no_alloc);
   }
   for (const RingDesc& d : desc) {
     if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
@@ -539,6 +589,10 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
     return PPM_ERR_DEVICE;
   }
   c->max_lds = max_dynamic_lds(o.device);
+  if ((e = noalloc_scratch_address(o.device, &c->noalloc))) {
+    ppm_close(c);
+    return PPM_ERR_DEVICE;
+  }
   {
     const char* ng = std::getenv("PPM_NO_GRAPH");
     c->no_graph = ng && *ng && *ng != '0';
@@ -680,6 +734,24 @@ ppm_status ppm_eval_device(ppm_ctx* c, const ppm_star* stars, int n, const ppm_g
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
   return eval_any(c, stars, n, grid, d_flux, d_counts, stats);
+}
+
+ppm_status ppm_eval_device_noalloc(ppm_ctx* c, const ppm_star* star, const ppm_grid* grid, double* d_flux,
+                                   double* d_counts, ppm_stats* stats) {
+  if (!c || !star || !grid || !d_flux) return fail(c, PPM_ERR_ARG, "null argument");
+  if (grid->method == PPM_METHOD_BATCHED)
+    return fail(c, PPM_ERR_ARG, "the no-allocation entry point evaluates on the plain path only");
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  hipSetDevice(c->device);
+  std::vector<double> energies;
+  ppm_status s = resolve_energies(c, *grid, energies);
+  if (s) return s;
+  if ((s = validate(c, *star, *grid, (int)energies.size()))) return s;
+  std::vector<PhasePoint> pts;
+  phase_points(*grid, pts);
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  return eval_plain_rows(c, *star, *grid, energies, pts, d_flux, grid->response >= 0 ? d_counts : nullptr, nullptr,
+                         stats, 0, true);
 }
 
 ppm_status ppm_eval(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, double* flux,

@@ -106,6 +106,10 @@ struct ppm_ctx {
   ppm::ServeState* serve = nullptr;
   ppm::BatchState* batch_state = nullptr;
   bool no_graph = false;  // PPM_NO_GRAPH=1: launch the batched sequence directly
+  // device address of this device's ppm_eval_device_noalloc scratch
+  // (kNoAllocScratchBytes), looked up at ppm_open; shared by every context
+  // on the device
+  void* noalloc = nullptr;
   // set by ppm_eval_fit for the duration of one evaluation: the folds also
   // accumulate fit statistics; fit_star is the first star of the next fold
   const ppm::FoldFit* fit = nullptr;

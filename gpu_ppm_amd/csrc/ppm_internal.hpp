@@ -149,6 +149,12 @@ int launch_fold(const uint16_t* rsp_hi, const uint16_t* rsp_lo, int n_ch, int n_
                 double* d_counts, void* stream, const FoldFit* fit = nullptr, int star0 = 0);
 int launch_fit_reduce(const double* d_part, int blocks, int n, double* d_out, void* stream);
 int max_dynamic_lds(int device);
+// Device scratch of ppm_eval_device_noalloc (a file-scope array in
+// ppm_kernels.hip, kNoAllocScratchBytes per GPU): its address on `device`,
+// looked up once per device.  Returns a hipError_t as int.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+constexpr int kMaxDevices = 64;
+int noalloc_scratch_address(int device, void** ptr);
 size_t integral_lds_bytes(int n_fine, int atmosphere, int nsx_rows);
 int energies_per_chunk();
 

@@ -15,11 +15,42 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "ppm_eval.hpp"
 #include "ppm_internal.hpp"
 
 namespace ppm {
+
+// Scratch of ppm_eval_device_noalloc: every device buffer of one plain
+// evaluation beyond the caller's inputs and outputs (spot and ring tables,
+// fine tables, per-ring partials, reduction groups, the fold's scale).  A
+// file-scope array, so that entry point calls no allocator; each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` is only a symbol: its device address comes from
+// noalloc_scratch_address.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch_address(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 namespace {
 

@@ -56,6 +56,9 @@ def load():
     L.ppm_eval_device.restype = c_int
     L.ppm_eval_device.argtypes = [c_void_p, POINTER(abi.Star), c_int, POINTER(abi.Grid), c_void_p, c_void_p,
                                   POINTER(abi.Stats)]
+    L.ppm_eval_device_noalloc.restype = c_int
+    L.ppm_eval_device_noalloc.argtypes = [c_void_p, POINTER(abi.Star), POINTER(abi.Grid), c_void_p, c_void_p,
+                                          POINTER(abi.Stats)]
     L.ppm_count_active.restype = c_int
     L.ppm_count_active.argtypes = [c_void_p, POINTER(abi.Star), POINTER(abi.Grid), POINTER(c_int64)]
     L.ppm_fold_device.restype = c_int
@@ -199,6 +202,14 @@ class Context:
         stats = abi.Stats()
         self._check(load().ppm_eval_device(self._h, arr, len(arr), byref(grid), c_void_p(d_flux),
                                            c_void_p(d_counts) if d_counts else None, byref(stats)))
+        return stats
+
+    def eval_device_noalloc(self, star, grid, d_flux, d_counts=None):
+        """One star on the plain path with no device allocation (the
+        library's fixed scratch); outputs to device pointers."""
+        stats = abi.Stats()
+        self._check(load().ppm_eval_device_noalloc(self._h, byref(star), byref(grid), c_void_p(d_flux),
+                                                   c_void_p(d_counts) if d_counts else None, byref(stats)))
         return stats
 
     def count_active(self, star, grid):

@@ -198,6 +198,21 @@ ppm_status ppm_eval(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* 
 ppm_status ppm_eval_device(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_grid* grid,
                            double* d_flux, double* d_counts, ppm_stats* stats);
 
+/* Same evaluation as ppm_eval_device for ONE star on the plain path, with
+ * no device-memory allocation: no counterpart in the reference; for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every device buffer beyond the caller's
+ * outputs lives in one file-scope scratch array of 128 MiB per GPU, held
+ * from library load until the process ends; a profile that needs more
+ * (about n_ring_tasks x n_phase x n_energy x 4 bytes of per-ring partials:
+ * N_side 1024, 64 bins, 360 energies fits; N_side 2048 x 128 bins does not)
+ * returns PPM_ERR_NOMEM before anything is launched.  Calls on one GPU share
+ * that array, from any context: calls that overlap on one GPU race (a ctx is
+ * used by one thread at a time anyway).  Outputs as ppm_eval_device;
+ * grid->method BATCHED is rejected. */
+ppm_status ppm_eval_device_noalloc(ppm_ctx* ctx, const ppm_star* star, const ppm_grid* grid, double* d_flux,
+                                   double* d_counts, ppm_stats* stats);
+
 /* Number of active patches of one star on a grid (FP64 membership on the
  * device).  Replaces the N_active print of GenerateHealpixGrid
  * (cpu/grid.hpp:247). */
