@@ -110,6 +110,13 @@ class TuneResult(ctypes.Structure):
                 ("reserved", c_int)]
 
 
+class ServeStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int64), ("running", ctypes.c_int64), ("launches", ctypes.c_int64 * 2),
+                ("overlapped", ctypes.c_int64), ("charged_bytes", ctypes.c_size_t),
+                ("budget_bytes", ctypes.c_size_t), ("device_bytes", ctypes.c_size_t),
+                ("fixed_bytes", ctypes.c_size_t)]
+
+
 NSX_H_V200804 = dict(n_logT=35, n_logg=14, n_logEkT=166, n_mu=67, logT_min=5.10, logT_max=6.80,
                      logg_min=13.7, logg_max=15.0, logEkT_min=-1.30, logEkT_max=2.00)
 # cpu/nsx.hpp:39-51

@@ -70,6 +70,30 @@ void HostBuf::release() {
   cap = 0;
 }
 
+int Lane::init() {
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) || (e = hipEventCreate(&ev0)) ||
+      (e = hipEventCreate(&ev1)) || (e = hipEventCreate(&ev2)) || (e = hipEventCreate(&ev_i0)) ||
+      (e = hipEventCreate(&ev_i1)) || (e = hipEventCreateWithFlags(&ev_staging, hipEventDisableTiming)))
+    return e;
+  return hipEventRecord(ev_staging, stream);
+}
+
+void Lane::release() {
+  if (stream) hipStreamSynchronize(stream);
+  ws.release_all();
+  batch.release_all();
+  release_lane_batch(*this);
+  staging.release();
+  for (hipEvent_t* ev : {&ev0, &ev1, &ev2, &ev_i0, &ev_i1, &ev_staging})
+    if (*ev) {
+      hipEventDestroy(*ev);
+      *ev = nullptr;
+    }
+  if (stream) hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
 // ------------------------------------------------------------ validation
 
 ppm_status validate(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, int n_e_resolved) {
@@ -262,23 +286,23 @@ ppm_status resolve_energies(ppm_ctx* c, const ppm_grid& g, std::vector<double>& 
   return PPM_OK;
 }
 
-int Upload::run(ppm_ctx* c) {
+int Upload::run(Lane& L) {
   size_t total = 0;
   for (auto& s : src) total += (s.second + 255) & ~(size_t)255;
-  int e = c->staging.ensure(total);
+  int e = L.staging.ensure(total);
   if (e) return e;
   // the previous upload from this buffer must have been consumed
-  e = hipEventSynchronize(c->ev_staging);
+  e = hipEventSynchronize(L.ev_staging);
   if (e) return e;
-  char* h = (char*)c->staging.p;
+  char* h = (char*)L.staging.p;
   size_t off = 0;
   for (size_t k = 0; k < src.size(); ++k) {
     std::memcpy(h + off, src[k].first, src[k].second);
-    e = hipMemcpyAsync(dst[k], h + off, src[k].second, hipMemcpyHostToDevice, c->stream);
+    e = hipMemcpyAsync(dst[k], h + off, src[k].second, hipMemcpyHostToDevice, L.stream);
     if (e) return e;
     off += (src[k].second + 255) & ~(size_t)255;
   }
-  return hipEventRecord(c->ev_staging, c->stream);
+  return hipEventRecord(L.ev_staging, L.stream);
 }
 
 // One star, plain path: the reference iteration space with one launch
@@ -298,14 +322,14 @@ int nsx_rows_for(const ppm_ctx* c, const std::vector<double>& energies) {
   return std::min(rows, c->N.nE);
 }
 
-ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
+ppm_status eval_plain_rows(ppm_ctx* c, Lane& L, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                            const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
                            double* d_scale_slot, ppm_stats* stats, int rows_override, bool no_alloc);
 
-ppm_status eval_plain(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
+ppm_status eval_plain(ppm_ctx* c, Lane& L, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                       const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
                       double* d_scale_slot, ppm_stats* stats) {
-  return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, 0, false);
+  return eval_plain_rows(c, L, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, 0, false);
 }
 
 // Device buffers of one plain evaluation, carved in order from one region
@@ -320,7 +344,7 @@ struct Carve {
   }
 };
 
-ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
+ppm_status eval_plain_rows(ppm_ctx* c, Lane& L, const ppm_star& st, const ppm_grid& g, const std::vector<double>& energies,
                            const std::vector<PhasePoint>& pts, double* d_flux, double* d_counts,
                            double* d_scale_slot, ppm_stats* stats, int rows_override, bool no_alloc) {
   const int n_e = (int)energies.size();
@@ -365,14 +389,14 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
     // host arrays copied directly (no pinned staging buffer, which would
     // have to be allocated); they stay alive until the stream is
     // synchronised below
-    if (((e = hipMemcpyAsync(p_spots, sp.data(), b_spots, hipMemcpyHostToDevice, c->stream))) ||
+    if (((e = hipMemcpyAsync(p_spots, sp.data(), b_spots, hipMemcpyHostToDevice, L.stream))) ||
         (n_tasks && (e = hipMemcpyAsync(p_tasks, tasks.data(), sizeof(RingTask) * n_tasks, hipMemcpyHostToDevice,
-                                        c->stream))) ||
-        (e = hipMemcpyAsync(p_pts, pts.data(), b_pts, hipMemcpyHostToDevice, c->stream)) ||
-        (e = hipMemcpyAsync(p_en, energies.data(), b_en, hipMemcpyHostToDevice, c->stream)))
+                                        L.stream))) ||
+        (e = hipMemcpyAsync(p_pts, pts.data(), b_pts, hipMemcpyHostToDevice, L.stream)) ||
+        (e = hipMemcpyAsync(p_en, energies.data(), b_en, hipMemcpyHostToDevice, L.stream)))
       return hip_fail(c, e, "upload");
   } else {
-    Workspace& W = c->ws;
+    Workspace& W = L.ws;
     if ((e = W.spots.ensure(b_spots)) || (e = W.tasks.ensure(b_tasks)) || (e = W.desc.ensure(b_desc)) ||
         (e = W.ph.ensure(b_ph)) || (e = W.fine.ensure(b_fine)) || (e = W.pts.ensure(b_pts)) ||
         (e = W.energies.ensure(b_en)) || (e = W.partial.ensure(b_part)) || (e = W.red.ensure(b_red)))
@@ -391,7 +415,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
     if (n_tasks) up.add(p_tasks, tasks.data(), sizeof(RingTask) * n_tasks);
     up.add(p_pts, pts.data(), b_pts);
     up.add(p_en, energies.data(), b_en);
-    if ((e = up.run(c))) return hip_fail(c, e, "upload");
+    if ((e = up.run(L))) return hip_fail(c, e, "upload");
   }
 
   PlainLaunch P{};
@@ -414,29 +438,29 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   P.emode = st.atmosphere == PPM_ATM_NSX ? nsx_energy_mode(st.nsx_interp) : kECubic;
   P.nsx_rows = nsx_rows;
 
-  if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");
-  if ((e = launch_ring_setup(c->L, c->N, P, c->stream))) return hip_fail(c, e, "ring set-up launch");
-  if ((e = hipEventRecord(c->ev_i0, c->stream))) return hip_fail(c, e, "event");
-  if ((e = launch_integral_plain(c->L, c->N, P, c->stream))) return hip_fail(c, e, "integral launch");
-  if ((e = hipEventRecord(c->ev_i1, c->stream))) return hip_fail(c, e, "event");
+  if ((e = hipEventRecord(L.ev0, L.stream))) return hip_fail(c, e, "event");
+  if ((e = launch_ring_setup(c->L, c->N, P, L.stream))) return hip_fail(c, e, "ring set-up launch");
+  if ((e = hipEventRecord(L.ev_i0, L.stream))) return hip_fail(c, e, "event");
+  if ((e = launch_integral_plain(c->L, c->N, P, L.stream))) return hip_fail(c, e, "integral launch");
+  if ((e = hipEventRecord(L.ev_i1, L.stream))) return hip_fail(c, e, "event");
   if (n_tasks == 0) {
-    if ((e = hipMemsetAsync(d_flux, 0, sizeof(double) * g.n_phase * n_e, c->stream)))
+    if ((e = hipMemsetAsync(d_flux, 0, sizeof(double) * g.n_phase * n_e, L.stream)))
       return hip_fail(c, e, "clear");
-  } else if ((e = launch_reduce(P, (double*)p_red, c->stream))) {
+  } else if ((e = launch_reduce(P, (double*)p_red, L.stream))) {
     return hip_fail(c, e, "reduce launch");
   }
-  if ((e = hipEventRecord(c->ev1, c->stream))) return hip_fail(c, e, "event");
+  if ((e = hipEventRecord(L.ev1, L.stream))) return hip_fail(c, e, "event");
   if (d_counts && g.response >= 0) {
     const Response& R = c->resp[g.response];
     const double scale = st.exposure_s * (1.0 / g.n_phase);  // cpu/devarshi24st_cpu.cpp:255
-    if ((e = hipMemcpyAsync(d_scale_slot, &scale, sizeof(double), hipMemcpyHostToDevice, c->stream)))
+    if ((e = hipMemcpyAsync(d_scale_slot, &scale, sizeof(double), hipMemcpyHostToDevice, L.stream)))
       return hip_fail(c, e, "scale upload");
-    if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, 1, g.n_phase, d_scale_slot, d_counts, c->stream,
-                         c->fit, c->fit_star)))
+    if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, 1, g.n_phase, d_scale_slot, d_counts, L.stream,
+                         L.fit, L.fit_star)))
       return hip_fail(c, e, "fold launch");
   }
-  if ((e = hipEventRecord(c->ev2, c->stream))) return hip_fail(c, e, "event");
-  if ((e = hipEventSynchronize(c->ev2))) return hip_fail(c, e, "execution");
+  if ((e = hipEventRecord(L.ev2, L.stream))) return hip_fail(c, e, "event");
+  if ((e = hipEventSynchronize(L.ev2))) return hip_fail(c, e, "execution");
   // read back the descriptors for statistics and run-overflow detection
   std::vector<RingDesc> desc(n_tasks);
   if (n_tasks && (e = hipMemcpy(desc.data(), p_desc, sizeof(RingDesc) * n_tasks, hipMemcpyDeviceToHost)))
@@ -447,7 +471,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   if (rows_overflow && nsx_rows < c->N.nE) {
     // a ring's redshift range needs more NSX rows than were staged: redo
     // with the whole slice (results of the first pass are discarded)
-    return eval_plain_rows(c, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, c->N.nE, no_alloc);
+    return eval_plain_rows(c, L, st, g, energies, pts, d_flux, d_counts, d_scale_slot, stats, c->N.nE, no_alloc);
   }
   for (const RingDesc& d : desc) {
     if (d.flags & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
@@ -457,9 +481,9 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   }
   if (stats) {
     float t01 = 0, t12 = 0, ti = 0;
-    hipEventElapsedTime(&t01, c->ev0, c->ev1);
-    hipEventElapsedTime(&t12, c->ev1, c->ev2);
-    hipEventElapsedTime(&ti, c->ev_i0, c->ev_i1);
+    hipEventElapsedTime(&t01, L.ev0, L.ev1);
+    hipEventElapsedTime(&t12, L.ev1, L.ev2);
+    hipEventElapsedTime(&ti, L.ev_i0, L.ev_i1);
     stats->kernel_ms += t01;
     stats->integral_ms += ti;
     stats->fold_ms += t12;
@@ -472,7 +496,7 @@ ppm_status eval_plain_rows(ppm_ctx* c, const ppm_star& st, const ppm_grid& g, co
   return PPM_OK;
 }
 
-ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
+ppm_status eval_any(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
                     double* d_counts, ppm_stats* stats) {
   if (!c || !stars || !grid || n < 1 || !d_flux) return fail(c, PPM_ERR_ARG, "null argument");
   std::vector<double> energies;
@@ -487,16 +511,16 @@ ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   int method = grid->method;
   if (method == PPM_METHOD_AUTO) method = n == 1 ? PPM_METHOD_PLAIN : PPM_METHOD_BATCHED;
   if (method == PPM_METHOD_BATCHED)
-    return eval_batched(c, stars, n, *grid, energies, pts, d_flux, d_counts, stats);
+    return eval_batched(c, L, stars, n, *grid, energies, pts, d_flux, d_counts, stats);
   if (method != PPM_METHOD_PLAIN) return fail(c, PPM_ERR_ARG, "unknown method");
-  int e = c->ws.scale.ensure(sizeof(double) * n);
+  int e = L.ws.scale.ensure(sizeof(double) * n);
   if (e) return hip_fail(c, e, "device allocation");
   const size_t fstride = (size_t)grid->n_phase * n_e;
   const size_t cstride = grid->response >= 0 ? (size_t)c->resp[grid->response].n_ch * grid->n_phase : 0;
   for (int i = 0; i < n; ++i) {
-    c->fit_star = i;
-    s = eval_plain(c, stars[i], *grid, energies, pts, d_flux + i * fstride,
-                   d_counts ? d_counts + i * cstride : nullptr, (double*)c->ws.scale.p + i, stats);
+    L.fit_star = i;
+    s = eval_plain(c, L, stars[i], *grid, energies, pts, d_flux + i * fstride,
+                   d_counts ? d_counts + i * cstride : nullptr, (double*)L.ws.scale.p + i, stats);
     if (s) return s;
   }
   return PPM_OK;
@@ -581,10 +605,8 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
   c->device = o.device;
   c->budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : ((size_t)16 << 30);
   int e;
-  if ((e = hipSetDevice(o.device)) || (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) ||
-      (e = hipEventCreate(&c->ev0)) || (e = hipEventCreate(&c->ev1)) || (e = hipEventCreate(&c->ev2)) ||
-      (e = hipEventCreate(&c->ev_i0)) || (e = hipEventCreate(&c->ev_i1)) ||
-      (e = hipEventCreateWithFlags(&c->ev_staging, hipEventDisableTiming)) || (e = hipEventRecord(c->ev_staging, c->stream))) {
+  if ((e = hipSetDevice(o.device)) || (e = c->main.init()) ||
+      (e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, o.device))) {
     ppm_close(c);
     return PPM_ERR_DEVICE;
   }
@@ -596,6 +618,8 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
   {
     const char* ng = std::getenv("PPM_NO_GRAPH");
     c->no_graph = ng && *ng && *ng != '0';
+    c->main.no_graph = c->no_graph;
+    c->main.acc_cap = std::min(kAccCapBytes, c->budget);
   }
   c->lens = std::move(lens);
   {  // lensing tables, FP64 on the device
@@ -691,22 +715,12 @@ ppm_status ppm_close(ppm_ctx* c) {
   if (!c) return PPM_ERR_ARG;
   serve_shutdown(c);
   hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  c->ws.release_all();
-  c->batch.release_all();
+  c->main.release();
   release_batch_state(c);
   c->d_lens.release();
   c->d_nsx.release();
   c->d_mu.release();
   for (auto& r : c->resp) r.buf.release();
-  c->staging.release();
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->ev2) hipEventDestroy(c->ev2);
-  if (c->ev_i0) hipEventDestroy(c->ev_i0);
-  if (c->ev_i1) hipEventDestroy(c->ev_i1);
-  if (c->ev_staging) hipEventDestroy(c->ev_staging);
-  if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return PPM_OK;
 }
@@ -733,7 +747,7 @@ ppm_status ppm_eval_device(ppm_ctx* c, const ppm_star* stars, int n, const ppm_g
   if (!c) return PPM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
-  return eval_any(c, stars, n, grid, d_flux, d_counts, stats);
+  return eval_any(c, c->main, stars, n, grid, d_flux, d_counts, stats);
 }
 
 ppm_status ppm_eval_device_noalloc(ppm_ctx* c, const ppm_star* star, const ppm_grid* grid, double* d_flux,
@@ -750,7 +764,8 @@ ppm_status ppm_eval_device_noalloc(ppm_ctx* c, const ppm_star* star, const ppm_g
   std::vector<PhasePoint> pts;
   phase_points(*grid, pts);
   if (stats) std::memset(stats, 0, sizeof *stats);
-  return eval_plain_rows(c, *star, *grid, energies, pts, d_flux, grid->response >= 0 ? d_counts : nullptr, nullptr,
+  return eval_plain_rows(c, c->main, *star, *grid, energies, pts, d_flux, grid->response >= 0 ? d_counts : nullptr,
+                         nullptr,
                          stats, 0, true);
 }
 
@@ -759,6 +774,7 @@ ppm_status ppm_eval(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   if (!c || !stars || !grid || n < 1 || !flux) return fail(c, PPM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
+  Lane& L = c->main;
   std::vector<double> energies;
   ppm_status s = resolve_energies(c, *grid, energies);
   if (s) return s;
@@ -766,13 +782,13 @@ ppm_status ppm_eval(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   const bool fold = counts && grid->response >= 0 && grid->response < (int)c->resp.size();
   const size_t nc = fold ? (size_t)n * c->resp[grid->response].n_ch * grid->n_phase : 0;
   int e;
-  if ((e = c->ws.out_flux.ensure(sizeof(double) * nf)) || (fold && (e = c->ws.out_counts.ensure(sizeof(double) * nc))))
+  if ((e = L.ws.out_flux.ensure(sizeof(double) * nf)) || (fold && (e = L.ws.out_counts.ensure(sizeof(double) * nc))))
     return hip_fail(c, e, "device allocation");
-  s = eval_any(c, stars, n, grid, (double*)c->ws.out_flux.p, fold ? (double*)c->ws.out_counts.p : nullptr, stats);
+  s = eval_any(c, L, stars, n, grid, (double*)L.ws.out_flux.p, fold ? (double*)L.ws.out_counts.p : nullptr, stats);
   if (s) return s;
-  if ((e = hipMemcpy(flux, c->ws.out_flux.p, sizeof(double) * nf, hipMemcpyDeviceToHost)))
+  if ((e = hipMemcpy(flux, L.ws.out_flux.p, sizeof(double) * nf, hipMemcpyDeviceToHost)))
     return hip_fail(c, e, "read-back");
-  if (fold && (e = hipMemcpy(counts, c->ws.out_counts.p, sizeof(double) * nc, hipMemcpyDeviceToHost)))
+  if (fold && (e = hipMemcpy(counts, L.ws.out_counts.p, sizeof(double) * nc, hipMemcpyDeviceToHost)))
     return hip_fail(c, e, "read-back");
   return PPM_OK;
 }
@@ -781,6 +797,7 @@ ppm_status ppm_count_active(ppm_ctx* c, const ppm_star* star, const ppm_grid* gr
   if (!c || !star || !grid || !n_active) return fail(c, PPM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
+  Lane& L = c->main;
   ppm_grid g = *grid;
   g.response = -1;
   g.n_fine = 4;
@@ -802,7 +819,7 @@ ppm_status ppm_count_active(ppm_ctx* c, const ppm_star* star, const ppm_grid* gr
   }
   *n_active = 0;
   if (tasks.empty()) return PPM_OK;
-  Workspace& W = c->ws;
+  Workspace& W = L.ws;
   int e;
   const int nt = (int)tasks.size();
   if ((e = W.spots.ensure(sizeof(SpotParams) * sp.size())) || (e = W.tasks.ensure(sizeof(RingTask) * nt)) ||
@@ -812,7 +829,7 @@ ppm_status ppm_count_active(ppm_ctx* c, const ppm_star* star, const ppm_grid* gr
   Upload up;
   up.add(W.spots.p, sp.data(), sizeof(SpotParams) * sp.size());
   up.add(W.tasks.p, tasks.data(), sizeof(RingTask) * nt);
-  if ((e = up.run(c))) return hip_fail(c, e, "upload");
+  if ((e = up.run(L))) return hip_fail(c, e, "upload");
   PlainLaunch P{};
   P.d_spots = (const SpotParams*)W.spots.p;
   P.d_tasks = (const RingTask*)W.tasks.p;
@@ -821,10 +838,10 @@ ppm_status ppm_count_active(ppm_ctx* c, const ppm_star* star, const ppm_grid* gr
   P.d_fine = (float*)W.fine.p;
   P.n_tasks = nt;
   P.n_fine = 4;
-  if ((e = launch_ring_setup(c->L, c->N, P, c->stream))) return hip_fail(c, e, "ring set-up launch");
+  if ((e = launch_ring_setup(c->L, c->N, P, L.stream))) return hip_fail(c, e, "ring set-up launch");
   std::vector<RingDesc> desc(nt);
-  if ((e = hipMemcpyAsync(desc.data(), W.desc.p, sizeof(RingDesc) * nt, hipMemcpyDeviceToHost, c->stream)) ||
-      (e = hipStreamSynchronize(c->stream)))
+  if ((e = hipMemcpyAsync(desc.data(), W.desc.p, sizeof(RingDesc) * nt, hipMemcpyDeviceToHost, L.stream)) ||
+      (e = hipStreamSynchronize(L.stream)))
     return hip_fail(c, e, "descriptor read-back");
   for (const RingDesc& d : desc) *n_active += d.n_act;
   return PPM_OK;
@@ -836,15 +853,16 @@ ppm_status ppm_fold_device(ppm_ctx* c, int response, int n_batch, int n_phase, c
   if (response < 0 || response >= (int)c->resp.size()) return fail(c, PPM_ERR_ARG, "response index out of range");
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
+  Lane& L = c->main;
   int e;
-  if ((e = c->ws.scale.ensure(sizeof(double) * n_batch))) return hip_fail(c, e, "device allocation");
-  if ((e = hipMemcpyAsync(c->ws.scale.p, scale, sizeof(double) * n_batch, hipMemcpyHostToDevice, c->stream)))
+  if ((e = L.ws.scale.ensure(sizeof(double) * n_batch))) return hip_fail(c, e, "device allocation");
+  if ((e = hipMemcpyAsync(L.ws.scale.p, scale, sizeof(double) * n_batch, hipMemcpyHostToDevice, L.stream)))
     return hip_fail(c, e, "upload");
   const Response& R = c->resp[response];
-  if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, n_batch, n_phase, (const double*)c->ws.scale.p,
-                       d_counts, c->stream)))
+  if ((e = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, d_flux, n_batch, n_phase, (const double*)L.ws.scale.p,
+                       d_counts, L.stream)))
     return hip_fail(c, e, "fold launch");
-  if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
+  if ((e = hipStreamSynchronize(L.stream))) return hip_fail(c, e, "execution");
   return PPM_OK;
 }
 
@@ -856,6 +874,7 @@ ppm_status ppm_eval_fit(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid
   if (data_per_star != 0 && data_per_star != 1) return fail(c, PPM_ERR_ARG, "data_per_star must be 0 or 1");
   std::lock_guard<std::mutex> lk(c->exec_mu);
   hipSetDevice(c->device);
+  Lane& L = c->main;
   std::vector<double> energies;
   ppm_status s = resolve_energies(c, *grid, energies);
   if (s) return s;
@@ -863,7 +882,7 @@ ppm_status ppm_eval_fit(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid
   const size_t per = (size_t)n_ch * grid->n_phase;
   const size_t nf = (size_t)n * grid->n_phase * energies.size(), nc = (size_t)n * per;
   const size_t nd = (data_per_star ? (size_t)n : 1) * per;
-  Workspace& W = c->ws;
+  Workspace& W = L.ws;
   int e;
   if ((e = W.out_flux.ensure(sizeof(double) * nf)) || (e = W.out_counts.ensure(sizeof(double) * nc)) ||
       (e = W.fit_data.ensure(sizeof(double) * nd)) || (e = W.fit_part.ensure(sizeof(double) * 3 * n * blocks)) ||
@@ -873,20 +892,20 @@ ppm_status ppm_eval_fit(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid
     return hip_fail(c, e, "data upload");
   const FoldFit F{(const double*)W.fit_data.p, data_per_star ? per : 0, (double*)W.fit_part.p, blocks, 0};
   struct Reset {
-    ppm_ctx* c;
+    Lane& L;
     ~Reset() {
-      c->fit = nullptr;
-      c->fit_star = 0;
+      L.fit = nullptr;
+      L.fit_star = 0;
     }
-  } reset{c};
-  c->fit = &F;
-  c->fit_star = 0;
-  if ((s = eval_any(c, stars, n, grid, (double*)W.out_flux.p, (double*)W.out_counts.p, stats))) return s;
+  } reset{L};
+  L.fit = &F;
+  L.fit_star = 0;
+  if ((s = eval_any(c, L, stars, n, grid, (double*)W.out_flux.p, (double*)W.out_counts.p, stats))) return s;
   std::vector<double> out(3 * (size_t)n);
-  if ((e = launch_fit_reduce((const double*)W.fit_part.p, blocks, n, (double*)W.fit_out.p, c->stream)) ||
-      (e = hipMemcpyAsync(out.data(), W.fit_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream)) ||
-      (counts && (e = hipMemcpyAsync(counts, W.out_counts.p, sizeof(double) * nc, hipMemcpyDeviceToHost, c->stream))) ||
-      (e = hipStreamSynchronize(c->stream)))
+  if ((e = launch_fit_reduce((const double*)W.fit_part.p, blocks, n, (double*)W.fit_out.p, L.stream)) ||
+      (e = hipMemcpyAsync(out.data(), W.fit_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, L.stream)) ||
+      (counts && (e = hipMemcpyAsync(counts, W.out_counts.p, sizeof(double) * nc, hipMemcpyDeviceToHost, L.stream))) ||
+      (e = hipStreamSynchronize(L.stream)))
     return hip_fail(c, e, "fit statistics");
   for (int b = 0; b < n; ++b) fit[b] = ppm_fit{out[3 * b], out[3 * b + 1], out[3 * b + 2]};
   return PPM_OK;

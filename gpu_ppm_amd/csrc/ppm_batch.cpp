@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 
 #include "ppm_batch_dev.hpp"
@@ -73,6 +74,7 @@ struct ClassCache {
   ClassTables tab;
   DevBuf csets, chdr, cpts;
   uint64_t last_use = 0;
+  int in_use = 0;  // launches in flight on some lane that read these tables
   void release() {
     csets.release();
     chdr.release();
@@ -88,9 +90,6 @@ struct GraphCache {
   std::vector<unsigned char> key, last_direct;
 };
 
-// Device memory of the item accumulators of one launch, at most.
-constexpr size_t kAccCapBytes = (size_t)12 << 30;
-
 // Launch shape of the batched kernel: work-queue items per resident
 // workgroup (how finely each spot's rings are split; more items balance the
 // persistent workgroups better, fewer carry less per-item set-up) and
@@ -104,17 +103,26 @@ constexpr int kDefaultItemsPerSlot = 10;
 constexpr int kTailChunks = 3;
 constexpr int kTailSplit = 4;
 
+// Caches shared by the lanes of a context (lanes run on several host
+// threads, so each map has its mutex).
 struct BatchState {
+  std::mutex cls_mu;
   std::map<std::vector<int>, ClassCache> classes;  // per (n_side, n_phase, sampling, n_gl)
   uint64_t use_clock = 0;
-  GraphCache graph;
-  DevBuf d_grids, d_order, d_star_item0, d_acc, d_seg;
-  int n_cu = 0;
   // shapes found by ppm_tune or set by the caller, per shape_key
+  std::mutex shape_mu;
   std::map<std::vector<int>, ppm_launch_shape> shapes;
-  const ppm_launch_shape* forced = nullptr;  // set while ppm_tune times a candidate
-  int occ_wg_max = 0;                        // occupancy limit of the last launch
 };
+
+}  // namespace ppm
+
+// Buffers and the graph of one lane's batched launches.
+struct ppm::LaneBatch {
+  ppm::GraphCache graph;
+  ppm::DevBuf d_grids, d_order, d_star_item0, d_acc, d_seg;
+};
+
+namespace ppm {
 
 // The grid fields that shape the batched work, the atmosphere and the bit
 // length of the star count.
@@ -125,8 +133,43 @@ static std::vector<int> shape_key(const ppm_grid& g, int n_e, int atm, int n) {
 }
 
 static BatchState* state(ppm_ctx* c) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
   if (!c->batch_state) c->batch_state = new BatchState();
   return c->batch_state;
+}
+
+static LaneBatch* lane_batch(Lane& L) {
+  if (!L.lb) L.lb = new LaneBatch();
+  return L.lb;
+}
+
+size_t lane_device_bytes(const Lane& L, size_t* fixed) {
+  size_t b = 0;
+  const Workspace& W = L.ws;
+  for (const DevBuf* d : {&W.spots, &W.tasks, &W.desc, &W.ph, &W.fine, &W.pts, &W.energies, &W.partial, &W.red,
+                          &W.scale, &W.out_flux, &W.out_counts, &W.fit_data, &W.fit_part, &W.fit_out})
+    b += d->cap;
+  const BatchWorkspace& B = L.batch;
+  for (const DevBuf* d : {&B.spots, &B.tasks, &B.desc, &B.ph, &B.fine, &B.pts, &B.energies, &B.items, &B.scale,
+                          &B.counter, &B.evals, &B.summary})
+    b += d->cap;
+  *fixed = 0;
+  if (L.lb) {
+    for (const DevBuf* d : {&L.lb->d_grids, &L.lb->d_order, &L.lb->d_star_item0, &L.lb->d_acc}) b += d->cap;
+    *fixed = L.lb->d_seg.cap;
+    b += *fixed;
+  }
+  return b;
+}
+
+void release_lane_batch(Lane& L) {
+  LaneBatch* B = L.lb;
+  if (!B) return;
+  if (B->graph.exec) hipGraphExecDestroy(B->graph.exec);
+  for (DevBuf* b : {&B->d_grids, &B->d_order, &B->d_star_item0, &B->d_acc, &B->d_seg}) b->release();
+  delete B;
+  L.lb = nullptr;
 }
 
 // Fraction of a ring of colatitude th inside a disk (th1, rho), for cost
@@ -153,16 +196,15 @@ static double ring_z(int n_side, int i) {
 void release_batch_state(ppm_ctx* c) {
   BatchState* S = c->batch_state;
   if (!S) return;
-  if (S->graph.exec) hipGraphExecDestroy(S->graph.exec);
   for (auto& kv : S->classes) kv.second.release();
-  for (DevBuf* b : {&S->d_grids, &S->d_order, &S->d_star_item0, &S->d_acc, &S->d_seg}) b->release();
   delete S;
   c->batch_state = nullptr;
 }
 
-static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats, int rows_override);
+static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats,
+                                   int rows_override);
 
-ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
+ppm_status eval_batched(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm_grid& g,
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats) {
   // one launch per launch class present, each over the runs of consecutive
@@ -194,15 +236,15 @@ ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid
       groups.push_back(std::move(G));
       i = j;
     }
-    const ppm_status s = eval_groups_rows(c, groups, stats, 0);
+    const ppm_status s = eval_groups_rows(c, L, groups, stats, 0);
     if (s) return s;
   }
   return PPM_OK;
 }
 
-ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats) {
+ppm_status eval_batched_groups(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats) {
   if (stats) std::memset(stats, 0, sizeof *stats);
-  return eval_groups_rows(c, groups, stats, 0);
+  return eval_groups_rows(c, L, groups, stats, 0);
 }
 
 // Class tables of a grid's phase points, built and uploaded once per grid
@@ -210,15 +252,17 @@ ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_
 // launch is dropped).
 constexpr size_t kMaxClassGrids = 24;
 
+// The caller holds BS->cls_mu; the entry returned is marked in use (the
+// caller releases it when its launch has completed).
 static ppm_status class_tables(ppm_ctx* c, BatchState* BS, const ppm_grid& g, const std::vector<PhasePoint>& pts,
-                               uint64_t launch_clock, ClassCache*& out) {
+                               ClassCache*& out) {
   const std::vector<int> key{g.n_side, g.n_phase, g.sampling, g.sampling == PPM_SAMPLE_GL ? g.n_gl : 0};
   auto it = BS->classes.find(key);
   if (it == BS->classes.end()) {
     if (BS->classes.size() >= kMaxClassGrids) {
       auto victim = BS->classes.end();
       for (auto v = BS->classes.begin(); v != BS->classes.end(); ++v)
-        if (v->second.last_use < launch_clock &&
+        if (v->second.in_use == 0 &&
             (victim == BS->classes.end() || v->second.last_use < victim->second.last_use))
           victim = v;
       if (victim != BS->classes.end()) {
@@ -243,6 +287,7 @@ static ppm_status class_tables(ppm_ctx* c, BatchState* BS, const ppm_grid& g, co
     it = BS->classes.find(key);
   }
   it->second.last_use = ++BS->use_clock;
+  ++it->second.in_use;
   out = &it->second;
   return PPM_OK;
 }
@@ -290,9 +335,11 @@ bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups) {
   return launch_geometry(c, groups, groups[0].stars[0].atmosphere, 0, r, f, w) <= (size_t)c->max_lds;
 }
 
-static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats, int rows_override) {
+static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats,
+                                   int rows_override) {
   BatchState* BS = state(c);
-  BatchWorkspace& W = c->batch;
+  LaneBatch* LB = lane_batch(L);
+  BatchWorkspace& W = L.batch;
   const int n_groups = (int)groups.size();
   int n = 0;  // stars of the launch
   for (const BatchGroup& G : groups) n += G.n;
@@ -308,17 +355,27 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
         return fail(c, PPM_ERR_ARG, "a launch needs one atmosphere and one energy-axis interpolant");
   }
   int e;
-  if (!BS->n_cu && hipDeviceGetAttribute(&BS->n_cu, hipDeviceAttributeMultiprocessorCount, c->device))
-    BS->n_cu = 256;
   const int epl = launch_epl(groups);
   const int per_item = batch_energies(epl);
 
-  // ---- class tables (per grid)
-  const uint64_t launch_clock = BS->use_clock + 1;
-  std::vector<ClassCache*> cls(n_groups);
-  for (int k = 0; k < n_groups; ++k) {
-    ppm_status s = class_tables(c, BS, groups[k].g, groups[k].pts, launch_clock, cls[k]);
-    if (s) return s;
+  // ---- class tables (per grid), held in use until this launch has ended
+  std::vector<ClassCache*> cls;
+  struct Release {
+    BatchState* BS;
+    std::vector<ClassCache*>* cls;
+    ~Release() {
+      std::lock_guard<std::mutex> lk(BS->cls_mu);
+      for (ClassCache* cc : *cls) --cc->in_use;
+    }
+  } release_cls{BS, &cls};
+  {
+    std::lock_guard<std::mutex> lk(BS->cls_mu);
+    for (int k = 0; k < n_groups; ++k) {
+      ClassCache* cc = nullptr;
+      ppm_status s = class_tables(c, BS, groups[k].g, groups[k].pts, cc);
+      if (s) return s;
+      cls.push_back(cc);
+    }
   }
 
   // ---- spots, ring tasks and cost estimates; the tasks of a grid are
@@ -374,23 +431,27 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   int nsx_rows, nf_max, cov_words;
   const size_t lds = launch_geometry(c, groups, atm, rows_override, nsx_rows, nf_max, cov_words);
   if (lds > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
-  if ((e = batched_prepare(atm, epl, emode, lds))) return hip_fail(c, e, "kernel attributes");
+  if ((e = batched_prepare(atm, epl, emode, c->max_lds))) return hip_fail(c, e, "kernel attributes");
   int wg_per_cu = 0;
   if (batched_occupancy(atm, epl, emode, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
-  BS->occ_wg_max = wg_per_cu;
+  L.occ_wg_max = wg_per_cu;
   ppm_launch_shape shape{0, 0};
-  if (BS->forced) {
-    shape = *BS->forced;
+  if (L.forced) {
+    shape = *L.forced;
   } else if (n_groups == 1) {
+    std::lock_guard<std::mutex> lk(BS->shape_mu);
     auto it = BS->shapes.find(shape_key(groups[0].g, (int)groups[0].energies.size(), atm, n));
     if (it != BS->shapes.end()) shape = it->second;
   }
   if (shape.wg_per_cu > 0) wg_per_cu = std::min(wg_per_cu, shape.wg_per_cu);
-  const int slots = wg_per_cu * BS->n_cu;
+  // workgroup slots left to the other lanes of the context
+  if (L.wg_leave > 0) wg_per_cu = std::max(1, wg_per_cu - L.wg_leave);
+  if (L.wg_max > 0) wg_per_cu = std::min(wg_per_cu, L.wg_max);
+  const int slots = wg_per_cu * c->n_cu;
 
   // ---- items: each spot's rings split into chunks of similar cost, times
   // the energy chunks of its grid; about per_slot items per resident
-  // workgroup, fewer if their accumulators would pass kAccCapBytes
+  // workgroup, fewer if their accumulators would pass the lane's cap
   // an item's accumulator copies cover its own energy chunk only
   std::vector<int> n_echunk(n_groups);
   for (int k = 0; k < n_groups; ++k) {
@@ -447,8 +508,15 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   };
   const double target = per_slot * slots;
   build_items(target);
-  if (acc_total * sizeof(double) > kAccCapBytes)
-    build_items(std::max(1.0, target * (double)kAccCapBytes / (double)(acc_total * sizeof(double))));
+  // fewer items until their accumulators fit the lane's cap (the rounding
+  // per spot and the finer tail make the count only roughly proportional)
+  for (double t = target; acc_total * sizeof(double) > L.acc_cap && t > 1.0;) {
+    t = std::max(1.0, t * std::min(0.9, 0.95 * (double)L.acc_cap / (double)(acc_total * sizeof(double))));
+    build_items(t);
+  }
+  if (acc_total * sizeof(double) > L.acc_cap)  // one item per spot and energy chunk is the least
+    return fail(c, PPM_ERR_NOMEM, "the accumulators of this launch (" + std::to_string(acc_total * sizeof(double)) +
+                                      " bytes at one item per spot) exceed the device-memory budget");
   star_item0[n] = (int)items.size();
   const int n_items = (int)items.size();
   std::vector<int> order(n_items);
@@ -476,13 +544,13 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       (e = W.fine.ensure(sizeof(float) * std::max<size_t>(n_fine_all, 1))) ||
       (e = W.energies.ensure(sizeof(double) * n_energy_all)) ||
       (e = W.items.ensure(sizeof(BatchItem) * std::max(n_items, 1))) ||
-      (e = BS->d_grids.ensure(sizeof(BatchGrid) * n_groups)) ||
-      (e = BS->d_order.ensure(sizeof(int) * std::max(n_items, 1))) ||
-      (e = BS->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) ||
+      (e = LB->d_grids.ensure(sizeof(BatchGrid) * n_groups)) ||
+      (e = LB->d_order.ensure(sizeof(int) * std::max(n_items, 1))) ||
+      (e = LB->d_star_item0.ensure(sizeof(int) * (n + 1))) || (e = W.counter.ensure(sizeof(int))) ||
       (e = W.evals.ensure(sizeof(unsigned long long))) ||
       (e = W.summary.ensure(sizeof(unsigned long long) * (4 * n_groups + 1))) ||
-      (e = BS->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
-      (e = BS->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
+      (e = LB->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
+      (e = LB->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
     return hip_fail(c, e, "device allocation");
   std::vector<BatchGrid> bg(n_groups);
@@ -508,14 +576,14 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
     up.add(W.spots.p, sp.data(), sizeof(SpotParams) * n_spots);
     if (n_tasks) up.add(W.tasks.p, tasks.data(), sizeof(RingTask) * n_tasks);
     up.add(W.energies.p, energies_all.data(), sizeof(double) * n_energy_all);
-    up.add(BS->d_grids.p, bg.data(), sizeof(BatchGrid) * n_groups);
+    up.add(LB->d_grids.p, bg.data(), sizeof(BatchGrid) * n_groups);
     if (n_items) {
       up.add(W.items.p, items.data(), sizeof(BatchItem) * n_items);
-      up.add(BS->d_order.p, order.data(), sizeof(int) * n_items);
+      up.add(LB->d_order.p, order.data(), sizeof(int) * n_items);
     }
-    up.add(BS->d_star_item0.p, star_item0.data(), sizeof(int) * (n + 1));
+    up.add(LB->d_star_item0.p, star_item0.data(), sizeof(int) * (n + 1));
     up.add(W.scale.p, fold_scale.data(), sizeof(double) * n);
-    if ((e = up.run(c))) return hip_fail(c, e, "upload");
+    if ((e = up.run(L))) return hip_fail(c, e, "upload");
   }
 
   // ring set-up per grid (its fine-table stride)
@@ -536,15 +604,15 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   B.spots = (const SpotParams*)W.spots.p;
   B.tasks = (const RingTask*)W.tasks.p;
   B.desc = (RingDesc*)W.desc.p;
-  B.grids = (const BatchGrid*)BS->d_grids.p;
+  B.grids = (const BatchGrid*)LB->d_grids.p;
   B.items = (const BatchItem*)W.items.p;
-  B.order = (const int*)BS->d_order.p;
-  B.star_item0 = (const int*)BS->d_star_item0.p;
+  B.order = (const int*)LB->d_order.p;
+  B.star_item0 = (const int*)LB->d_star_item0.p;
   B.queue = (int*)W.counter.p;
   B.n_evals = (unsigned long long*)W.evals.p;
   B.rows_flag = (unsigned long long*)W.summary.p + 4 * n_groups;
-  B.acc = (double*)BS->d_acc.p;
-  B.seg = (double*)BS->d_seg.p;
+  B.acc = (double*)LB->d_acc.p;
+  B.seg = (double*)LB->d_seg.p;
   B.n_items = n_items;
   B.n_fine_max = nf_max;
   B.nsx_rows = nsx_rows;
@@ -566,7 +634,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   put(&lds, sizeof lds);
   {
     const FoldFit none{};
-    put(c->fit ? c->fit : &none, sizeof(FoldFit));
+    put(L.fit ? L.fit : &none, sizeof(FoldFit));
   }
   for (int k = 0; k < n_groups; ++k) {
     put(&P[k], sizeof(PlainLaunch));
@@ -581,31 +649,31 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   // as a whole by events recorded around hipGraphLaunch (outside the graph)
   auto record = [&](bool timed) -> int {
     int r;
-    if (timed && (r = hipEventRecord(c->ev0, c->stream))) return r;
-    if ((r = launch_zero_u64((unsigned long long*)W.summary.p, 4 * n_groups + 1, c->stream))) return r;
+    if (timed && (r = hipEventRecord(L.ev0, L.stream))) return r;
+    if ((r = launch_zero_u64((unsigned long long*)W.summary.p, 4 * n_groups + 1, L.stream))) return r;
     for (int k = 0; k < n_groups; ++k)
-      if (P[k].n_tasks && (r = launch_ring_setup(c->L, c->N, P[k], c->stream))) return r;
-    if (timed && (r = hipEventRecord(c->ev_i0, c->stream))) return r;
-    if (n_items && (r = launch_batched(c->N, B, n_wg, c->stream))) return r;
-    if (timed && (r = hipEventRecord(c->ev_i1, c->stream))) return r;
+      if (P[k].n_tasks && (r = launch_ring_setup(c->L, c->N, P[k], L.stream))) return r;
+    if (timed && (r = hipEventRecord(L.ev_i0, L.stream))) return r;
+    if (n_items && (r = launch_batched(c->N, B, n_wg, L.stream))) return r;
+    if (timed && (r = hipEventRecord(L.ev_i1, L.stream))) return r;
     for (int k = 0; k < n_groups; ++k)
-      if ((r = launch_reduce_batched(B, bg[k], group_star0[k], group_star0[k + 1], groups[k].d_flux, c->stream)))
+      if ((r = launch_reduce_batched(B, bg[k], group_star0[k], group_star0[k + 1], groups[k].d_flux, L.stream)))
         return r;
-    if (timed && (r = hipEventRecord(c->ev1, c->stream))) return r;
+    if (timed && (r = hipEventRecord(L.ev1, L.stream))) return r;
     for (int k = 0; k < n_groups; ++k) {
       const BatchGroup& G = groups[k];
       if (!G.d_counts || G.g.response < 0) continue;
       const Response& R = c->resp[G.g.response];
       if ((r = launch_fold(R.d_hi, R.d_lo, R.n_ch, R.n_e, G.d_flux, G.n, G.g.n_phase,
-                           (const double*)W.scale.p + group_star0[k], G.d_counts, c->stream, c->fit,
+                           (const double*)W.scale.p + group_star0[k], G.d_counts, L.stream, L.fit,
                            G.fit_star0 >= 0 ? G.fit_star0 : group_star0[k])))
         return r;
     }
-    if (timed && (r = hipEventRecord(c->ev2, c->stream))) return r;
+    if (timed && (r = hipEventRecord(L.ev2, L.stream))) return r;
     return hipSuccess;
   };
-  GraphCache& GC = BS->graph;
-  const bool use_graph = !c->no_graph && ((GC.exec && GC.key == key) || GC.last_direct == key);
+  GraphCache& GC = LB->graph;
+  const bool use_graph = !L.no_graph && ((GC.exec && GC.key == key) || GC.last_direct == key);
   if (use_graph) {
     if (!GC.exec || GC.key != key) {
       if (GC.exec) {
@@ -613,9 +681,9 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
         GC.exec = nullptr;
       }
       hipGraph_t graph = nullptr;
-      if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal))) return hip_fail(c, e, "capture");
+      if ((e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal))) return hip_fail(c, e, "capture");
       const int r = record(false);
-      e = hipStreamEndCapture(c->stream, &graph);
+      e = hipStreamEndCapture(L.stream, &graph);
       if (r) {
         if (graph) hipGraphDestroy(graph);
         return hip_fail(c, r, "batched launch");
@@ -629,20 +697,20 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
       }
       GC.key = key;
     }
-    if ((e = hipEventRecord(c->ev0, c->stream))) return hip_fail(c, e, "event");
-    if ((e = hipGraphLaunch(GC.exec, c->stream))) return hip_fail(c, e, "graph launch");
-    if ((e = hipEventRecord(c->ev2, c->stream))) return hip_fail(c, e, "event");
+    if ((e = hipEventRecord(L.ev0, L.stream))) return hip_fail(c, e, "event");
+    if ((e = hipGraphLaunch(GC.exec, L.stream))) return hip_fail(c, e, "graph launch");
+    if ((e = hipEventRecord(L.ev2, L.stream))) return hip_fail(c, e, "event");
   } else {
     if ((e = record(true))) return hip_fail(c, e, "batched launch");
     GC.last_direct = key;
   }
-  if ((e = hipStreamSynchronize(c->stream))) return hip_fail(c, e, "execution");
+  if ((e = hipStreamSynchronize(L.stream))) return hip_fail(c, e, "execution");
 
   // the ring set-up's counters per grid and the integral's row-overflow flag
   std::vector<unsigned long long> sum(4 * n_groups + 1);
   if ((e = hipMemcpy(sum.data(), W.summary.p, sizeof(unsigned long long) * sum.size(), hipMemcpyDeviceToHost)))
     return hip_fail(c, e, "summary read-back");
-  if ((sum[4 * n_groups] & 2) && nsx_rows < c->N.nE) return eval_groups_rows(c, groups, stats, c->N.nE);
+  if ((sum[4 * n_groups] & 2) && nsx_rows < c->N.nE) return eval_groups_rows(c, L, groups, stats, c->N.nE);
   int64_t n_act = 0, n_eval_ref = 0, n_rings = 0;
   for (int k = 0; k < n_groups; ++k) {
     if (sum[4 * k + 3] & 1) return fail(c, PPM_ERR_UNSUPPORTED, "a ring has more active runs than supported");
@@ -652,13 +720,13 @@ static ppm_status eval_groups_rows(ppm_ctx* c, std::vector<BatchGroup>& groups, 
   }
   if (stats) {
     float t02 = 0;
-    hipEventElapsedTime(&t02, c->ev0, c->ev2);
+    hipEventElapsedTime(&t02, L.ev0, L.ev2);
     stats->total_ms += t02;
     if (!use_graph) {  // per-kernel times exist only for direct launches
       float t01 = 0, t12 = 0, ti = 0;
-      hipEventElapsedTime(&t01, c->ev0, c->ev1);
-      hipEventElapsedTime(&t12, c->ev1, c->ev2);
-      hipEventElapsedTime(&ti, c->ev_i0, c->ev_i1);
+      hipEventElapsedTime(&t01, L.ev0, L.ev1);
+      hipEventElapsedTime(&t12, L.ev1, L.ev2);
+      hipEventElapsedTime(&ti, L.ev_i0, L.ev_i1);
       stats->kernel_ms += t01;
       stats->integral_ms += ti;
       stats->fold_ms += t12;
@@ -721,30 +789,30 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
     return hip_fail(c, e, "device allocation");
   }
   BatchState* BS = state(c);
+  Lane& L = c->main;
   ppm_launch_shape cur{0, 0};
   // direct launches (the integral's own events) and the candidate shape,
   // restored on every exit
   struct Restore {
-    ppm_ctx* c;
-    BatchState* BS;
+    Lane& L;
     DevBuf* buf;
     bool no_graph;
     ~Restore() {
-      c->no_graph = no_graph;
-      BS->forced = nullptr;
-      hipStreamSynchronize(c->stream);
+      L.no_graph = no_graph;
+      L.forced = nullptr;
+      hipStreamSynchronize(L.stream);
       buf->release();
     }
-  } restore{c, BS, &d_flux, c->no_graph};
-  c->no_graph = true;
-  BS->forced = &cur;
+  } restore{L, &d_flux, L.no_graph};
+  L.no_graph = true;
+  L.forced = &cur;
   int n_tried = 0;
   auto time_shape = [&](ppm_launch_shape sh, double& ms) -> ppm_status {
     cur = sh;
     ms = 1e300;
     for (int r = 0; r < repeats; ++r) {
       ppm_stats st{};
-      const ppm_status q = eval_any(c, stars, n, &g, (double*)d_flux.p, nullptr, &st);
+      const ppm_status q = eval_any(c, L, stars, n, &g, (double*)d_flux.p, nullptr, &st);
       if (q) return q;
       ms = std::min(ms, st.integral_ms);
     }
@@ -756,9 +824,9 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   {
     cur = ppm_launch_shape{0, 0};
     ppm_stats st{};
-    if ((s = eval_any(c, stars, n, &g, (double*)d_flux.p, nullptr, &st))) return s;
+    if ((s = eval_any(c, L, stars, n, &g, (double*)d_flux.p, nullptr, &st))) return s;
   }
-  const int wg_max = std::max(1, BS->occ_wg_max);
+  const int wg_max = std::max(1, L.occ_wg_max);
   ppm_launch_shape best{kDefaultItemsPerSlot, wg_max};
   double t_best, t_def;
   if ((s = time_shape(best, t_def))) return s;
@@ -782,7 +850,10 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   }
   // one resident workgroup fewer per CU
   if (wg_max > 1 && (s = consider(ppm_launch_shape{best.items_per_slot, wg_max - 1}))) return s;
-  BS->shapes[key] = best;
+  {
+    std::lock_guard<std::mutex> lk(BS->shape_mu);
+    BS->shapes[key] = best;
+  }
   out->best = best;
   out->best_ms = t_best;
   out->default_ms = t_def;
@@ -802,6 +873,7 @@ ppm_status ppm_set_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, i
   ppm_status s = shape_args(c, grid, n_stars, atmosphere, key);
   if (s) return s;
   BatchState* BS = state(c);
+  std::lock_guard<std::mutex> lk2(BS->shape_mu);
   if (shape && (shape->items_per_slot || shape->wg_per_cu))
     BS->shapes[key] = *shape;
   else
@@ -817,6 +889,7 @@ ppm_status ppm_get_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, i
   ppm_status s = shape_args(c, grid, n_stars, atmosphere, key);
   if (s) return s;
   BatchState* BS = state(c);
+  std::lock_guard<std::mutex> lk2(BS->shape_mu);
   auto it = BS->shapes.find(key);
   *shape = it != BS->shapes.end() ? it->second : ppm_launch_shape{0, 0};
   return PPM_OK;

@@ -15,7 +15,7 @@ inline int batch_emode(const ppm_star& s) {
 }
 inline int batch_class(const ppm_star& s) { return s.atmosphere == PPM_ATM_NSX ? 1 + batch_emode(s) : 0; }
 
-ppm_status eval_batched(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid& g,
+ppm_status eval_batched(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm_grid& g,
                         const std::vector<double>& energies, const std::vector<PhasePoint>& pts,
                         double* d_flux, double* d_counts, ppm_stats* stats);
 
@@ -33,7 +33,7 @@ struct BatchGroup {
 
 // One persistent launch over the stars of several grids (one launch class,
 // energies strictly increasing in every group).
-ppm_status eval_batched_groups(ppm_ctx* c, std::vector<BatchGroup>& groups, ppm_stats* stats);
+ppm_status eval_batched_groups(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats);
 
 // Whether one launch over these groups fits the LDS of a workgroup.
 bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups);

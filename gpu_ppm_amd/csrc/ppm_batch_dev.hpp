@@ -73,7 +73,7 @@ int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 int batch_energies(int epl);  // energies per item
-int batched_prepare(int atmosphere, int epl, int emode, size_t lds);  // outside any stream capture
+int batched_prepare(int atmosphere, int epl, int emode, int lds_max);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);
 int launch_zero_u64(unsigned long long* p, int n, void* stream);  // a kernel, not a memset (graph capture)
 // flux[s - s0][b][e] for the stars [s0, s1) of one grid (G, its host copy).

@@ -582,8 +582,14 @@ const void* batched_fn(int atmosphere, int epl, int emode) {
 }
 }  // namespace
 
-int batched_prepare(int atmosphere, int epl, int emode, size_t lds) {
-  return hipFuncSetAttribute(batched_fn(atmosphere, epl, emode), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+// The dynamic LDS limit is set to the device's maximum (lds_max), the same
+// value from every lane, so concurrent launches never lower it for another.
+int batched_prepare(int atmosphere, int epl, int emode, int lds_max) {
+  const void* fn = batched_fn(atmosphere, epl, emode);
+  hipFuncAttributes fa{};
+  const hipError_t e = hipFuncGetAttributes(&fa, fn);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max - (int)fa.sharedSizeBytes);
 }
 
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream) {

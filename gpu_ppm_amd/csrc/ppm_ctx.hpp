@@ -63,7 +63,36 @@ struct Response {
 };
 
 struct ServeState;  // ppm_serve.cpp
-struct BatchState;  // ppm_batch.cpp
+struct BatchState;  // ppm_batch.cpp: caches shared by the lanes of a context
+struct LaneBatch;   // ppm_batch.cpp: buffers of one lane's batched launches
+
+// Accumulator bytes of one batched launch at most (further capped by the
+// context's device-memory budget).
+constexpr size_t kAccCapBytes = (size_t)12 << 30;
+
+// A lane: one stream of device work with its own buffers, so that lanes run
+// concurrently on the device.  The context's main lane serves the blocking
+// entry points (one host thread at a time); each serving worker owns one.
+struct Lane {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_i0 = nullptr, ev_i1 = nullptr, ev_staging = nullptr;
+  HostBuf staging;  // pinned host memory of uploads (free again after ev_staging)
+  Workspace ws;
+  BatchWorkspace batch;
+  LaneBatch* lb = nullptr;
+  // fold-epilogue fit statistics for the duration of one ppm_eval_fit call
+  // (main lane only); fit_star is the first star of the next fold
+  const FoldFit* fit = nullptr;
+  int fit_star = 0;
+  bool no_graph = false;                     // launch the batched sequence directly
+  const ppm_launch_shape* forced = nullptr;  // shape under test by ppm_tune
+  size_t acc_cap = kAccCapBytes;             // accumulator bytes of one batched launch
+  int wg_leave = 0;  // resident workgroups per CU the batched launches leave to other lanes
+  int wg_max = 0;    // resident workgroups per CU of the next batched launch at most (0: no cap)
+  int occ_wg_max = 0;  // occupancy limit of the last batched launch
+  int init();
+  void release();
+};
 
 inline uint16_t bf16_round(float f) {
   uint32_t u;
@@ -84,9 +113,9 @@ inline float bf16_value(uint16_t h) {
 struct ppm_ctx {
   int device = 0;
   int max_lds = 65536;
+  int n_cu = 256;
   size_t budget = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_i0 = nullptr, ev_i1 = nullptr, ev_staging = nullptr;
+  ppm::Lane main;      // the blocking entry points' stream and buffers
   std::string err;
   std::mutex err_mu;
   std::mutex exec_mu;  // one evaluation on the device at a time
@@ -99,10 +128,6 @@ struct ppm_ctx {
   std::vector<double> nsx_mu;
   std::vector<ppm::Response> resp;
 
-  ppm::Workspace ws;
-  ppm::BatchWorkspace batch;
-  ppm::HostBuf staging;
-
   ppm::ServeState* serve = nullptr;
   ppm::BatchState* batch_state = nullptr;
   bool no_graph = false;  // PPM_NO_GRAPH=1: launch the batched sequence directly
@@ -110,10 +135,6 @@ struct ppm_ctx {
   // (kNoAllocScratchBytes), looked up at ppm_open; shared by every context
   // on the device
   void* noalloc = nullptr;
-  // set by ppm_eval_fit for the duration of one evaluation: the folds also
-  // accumulate fit statistics; fit_star is the first star of the next fold
-  const ppm::FoldFit* fit = nullptr;
-  int fit_star = 0;
 };
 
 namespace ppm {
@@ -125,15 +146,18 @@ void spot_params(const ppm_ctx* c, const ppm_star& st, const ppm_spot& sp, int n
 void candidate_rings(const SpotParams& P, const ppm_spot& sp, int n_side, std::vector<int>& rings);
 void phase_points(const ppm_grid& g, std::vector<PhasePoint>& pts);
 ppm_status resolve_energies(ppm_ctx* c, const ppm_grid& g, std::vector<double>& e);
-ppm_status eval_any(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
+ppm_status eval_any(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
                     double* d_counts, ppm_stats* stats);
 
 void serve_init(ppm_ctx* c);
 void serve_shutdown(ppm_ctx* c);
 void release_batch_state(ppm_ctx* c);
+void release_lane_batch(Lane& L);
+// Device bytes held by a lane's buffers; *fixed receives its segment-total scratch.
+size_t lane_device_bytes(const Lane& L, size_t* fixed);
 
-// Host arrays copied to the device through the context's pinned staging
-// buffer on its stream (ppm_api.cpp).
+// Host arrays copied to the device through a lane's pinned staging buffer on
+// its stream (ppm_api.cpp).
 struct Upload {
   std::vector<std::pair<const void*, size_t>> src;
   std::vector<void*> dst;
@@ -141,7 +165,7 @@ struct Upload {
     src.emplace_back(s, n);
     dst.push_back(d);
   }
-  int run(ppm_ctx* c);
+  int run(Lane& L);
 };
 
 }  // namespace ppm

@@ -862,8 +862,14 @@ int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P
 template <int ATM, int EM>
 static hipError_t launch_plain_inst(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, dim3 grid, size_t lds,
                                     hipStream_t st) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_integral_plain<ATM, EM>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  // the limit is set to the device's maximum, the same value from every lane
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipFuncAttributes fa{};
+  if ((e = hipFuncGetAttributes(&fa, (const void*)k_integral_plain<ATM, EM>)) != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_integral_plain<ATM, EM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          max_dynamic_lds(dev) - (int)fa.sharedSizeBytes);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_integral_plain<ATM, EM>), grid, dim3(kBlock), lds, st, L, N, P);
   return hipGetLastError();

@@ -1,24 +1,33 @@
-// Serving loop: continuous batching of single-profile requests.
+// Serving loop: continuous batching of single-profile requests, with work
+// of requests that arrive while others run overlapping on the device.
 //
 // ppm_submit validates a request, copies its star, grid and energy grid,
-// charges its device memory against the context's budget and queues it.
-// A worker thread owned by the context takes, whenever the device is free,
-// every queued request whose launch class (atmosphere and, for NSX, the
-// interpolant's energy-axis evaluation) is the oldest request's (up to
-// kMaxBatch, in arrival order), groups them by grid and evaluates them in
-// persistent batched launches (several grids per launch, the cheapest launch
-// first), while new requests keep arriving.  The results of a launch are
-// copied to host memory of its requests as soon as it ends and handed out by
-// ppm_poll in any order.  The reference has no
-// serving layer; each request is one calc_one_star
-// (gpu/devarshi24st_gpu.cu:431-511) without the text output.
+// estimates its work and device memory, charges the memory against the
+// context's budget and queues it.  Two workers, each with its own lane
+// (stream and buffers), take queued requests whenever they are free:
+//   - the bulk worker takes any request; its persistent launches leave one
+//     resident workgroup per CU free;
+//   - the express worker takes light requests (estimated work below
+//     kLightWork); while the bulk worker has a launch in flight its launches
+//     run in the free workgroup per CU, concurrently with that launch, so a
+//     light request that arrives during a heavy launch does not wait for it.
+// A worker takes every queued request it may serve of the oldest such
+// request's launch class (atmosphere and, for NSX, the interpolant's
+// energy-axis evaluation), up to its lane's memory share, groups them by
+// grid and evaluates them in persistent batched launches (several grids per
+// launch, the cheapest launch first).  The results of a launch are copied to
+// host memory of its requests as soon as it ends and handed out by ppm_poll
+// in any order.  The reference has no serving layer; each request is one
+// calc_one_star (gpu/devarshi24st_gpu.cu:431-511) without the text output.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <memory>
 #include <numeric>
 #include <unordered_map>
 
 #include "ppm_batch.hpp"
+#include "ppm_batch_dev.hpp"
 #include "ppm_ctx.hpp"
 
 namespace ppm {
@@ -30,6 +39,8 @@ using Clock = std::chrono::steady_clock;
 constexpr int kMaxBatch = 1024;          // requests per launch sequence
 constexpr size_t kMaxQueued = 1 << 16;   // queued + running requests
 constexpr size_t kDefaultBudget = (size_t)16 << 30;
+constexpr double kLightWork = 1e11;      // reference-space evaluations of a light request
+enum { kExpress = 0, kBulk = 1, kWorkers = 2 };
 
 enum class ReqState { kQueued, kRunning, kDone, kFailed, kCancelled };
 
@@ -38,7 +49,9 @@ struct Request {
   ppm_grid grid{};
   std::vector<double> energies;  // the grid's energies, owned here
   int n_ch = 0;                  // channels of the fold, 0 without
-  size_t bytes = 0;              // device memory charged at admission
+  double work = 0;               // estimated evaluations in the reference iteration space
+  size_t bytes = 0;              // device memory charged at admission (outputs and ring tables)
+  size_t acc_bytes = 0;          // accumulators at one work item per spot and energy chunk
   std::vector<double> flux, counts;
   Clock::time_point t_submit, t_done;
   ReqState state = ReqState::kQueued;
@@ -56,6 +69,16 @@ bool same_grid(const Request& a, const Request& b) {
 
 }  // namespace
 
+struct ServeWorker {
+  Lane lane;
+  std::thread thread;
+  bool busy = false;
+  DevBuf d_flux, d_counts;
+  HostBuf h_flux, h_counts;
+  int64_t launches = 0, overlapped = 0;
+  size_t held = 0, held_fixed = 0;  // device bytes of the lane's buffers after its last launch
+};
+
 struct ServeState {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -63,18 +86,26 @@ struct ServeState {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_ticket = 1;
   size_t budget = kDefaultBudget, charged = 0;
-  int active = 0;  // queued + running
-  bool stop = false;
-  std::thread worker;
-  // launch buffers of the worker
-  DevBuf d_flux, d_counts;
-  HostBuf h_flux, h_counts;
+  size_t lane_share = 0;  // per worker: accumulator cap, and request bytes of one launch at most
+  int active = 0;         // queued + running
+  bool stop = false, started = false;
+  ServeWorker workers[kWorkers];
 };
 
 namespace {
 
-void worker_loop(ppm_ctx* c) {
+// Whether worker w may take request r now (S.mu held).
+bool eligible(const ServeState& S, int w, const Request& r) {
+  const bool light = r.work < kLightWork;
+  if (w == kExpress) return light;
+  return !light || S.workers[kExpress].busy;  // light requests go to the bulk lane when express is busy
+}
+
+void worker_loop(ppm_ctx* c, int w) {
   ServeState& S = *c->serve;
+  ServeWorker& W = S.workers[w];
+  Lane& L = W.lane;
+  hipSetDevice(c->device);
   std::vector<Request*> batch;
   std::vector<uint64_t> tickets;
   std::vector<std::vector<int>> members;  // per grid group: indices into batch
@@ -84,13 +115,24 @@ void worker_loop(ppm_ctx* c) {
     tickets.clear();
     {
       std::unique_lock<std::mutex> lk(S.mu);
-      S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty(); });
+      auto first = S.queue.end();
+      S.cv_work.wait(lk, [&] {
+        if (S.stop) return true;
+        first = std::find_if(S.queue.begin(), S.queue.end(),
+                             [&](uint64_t t) { return eligible(S, w, *S.reqs.at(t)); });
+        return first != S.queue.end();
+      });
       if (S.stop) return;
-      // the oldest request and every queued request of its launch class
-      const int cls = batch_class(S.reqs.at(S.queue.front())->star);
-      for (auto it = S.queue.begin(); it != S.queue.end() && (int)batch.size() < kMaxBatch;) {
+      // the oldest request this worker may serve and every queued one of
+      // its launch class, within the lane's memory share
+      const int cls = batch_class(S.reqs.at(*first)->star);
+      size_t acc = 0, bytes = 0;
+      for (auto it = first; it != S.queue.end() && (int)batch.size() < kMaxBatch;) {
         Request* r = S.reqs.at(*it).get();
-        if (batch_class(r->star) == cls) {
+        if (batch_class(r->star) == cls && eligible(S, w, *r) &&
+            (batch.empty() || (acc + r->acc_bytes <= S.lane_share && bytes + r->bytes <= S.lane_share))) {
+          acc += r->acc_bytes;
+          bytes += r->bytes;
           r->state = ReqState::kRunning;
           batch.push_back(r);
           tickets.push_back(*it);
@@ -99,6 +141,17 @@ void worker_loop(ppm_ctx* c) {
           ++it;
         }
       }
+      W.busy = true;
+      // workgroup slots: bulk launches always leave one per CU free; an
+      // express launch takes that one only while a bulk launch runs, else
+      // the whole device
+      if (w == kBulk) {
+        L.wg_leave = 1;
+      } else {
+        L.wg_max = S.workers[kBulk].busy ? 1 : 0;
+        W.overlapped += S.workers[kBulk].busy;
+      }
+      ++W.launches;
     }
     const int n = (int)batch.size();
     // group by grid, groups in order of their first request
@@ -130,7 +183,7 @@ void worker_loop(ppm_ctx* c) {
           for (size_t j = 0; j < members[gi].size(); ++j) {
             const int k = members[gi][j];
             Request* r = batch[k];
-            S.charged -= r->bytes;
+            S.charged -= r->bytes + r->acc_bytes;
             --S.active;
             if (r->cancel_pending) {
               S.reqs.erase(tickets[k]);
@@ -139,10 +192,10 @@ void worker_loop(ppm_ctx* c) {
             r->t_done = now;
             r->status = st[gi];
             if (st[gi] == PPM_OK) {
-              const double* f = (const double*)S.h_flux.p + f_off[gi] + per_flux * j;
+              const double* f = (const double*)W.h_flux.p + f_off[gi] + per_flux * j;
               r->flux.assign(f, f + per_flux);
               if (per_counts) {
-                const double* q = (const double*)S.h_counts.p + c_off[gi] + per_counts * j;
+                const double* q = (const double*)W.h_counts.p + c_off[gi] + per_counts * j;
                 r->counts.assign(q, q + per_counts);
               }
               r->state = ReqState::kDone;
@@ -160,13 +213,13 @@ void worker_loop(ppm_ctx* c) {
       for (int gi : gis) {
         if (st[gi] != PPM_OK) continue;
         if (!e && f_off[gi + 1] > f_off[gi])
-          e = hipMemcpyAsync((double*)S.h_flux.p + f_off[gi], (const double*)S.d_flux.p + f_off[gi],
-                             sizeof(double) * (f_off[gi + 1] - f_off[gi]), hipMemcpyDeviceToHost, c->stream);
+          e = hipMemcpyAsync((double*)W.h_flux.p + f_off[gi], (const double*)W.d_flux.p + f_off[gi],
+                             sizeof(double) * (f_off[gi + 1] - f_off[gi]), hipMemcpyDeviceToHost, L.stream);
         if (!e && c_off[gi + 1] > c_off[gi])
-          e = hipMemcpyAsync((double*)S.h_counts.p + c_off[gi], (const double*)S.d_counts.p + c_off[gi],
-                             sizeof(double) * (c_off[gi + 1] - c_off[gi]), hipMemcpyDeviceToHost, c->stream);
+          e = hipMemcpyAsync((double*)W.h_counts.p + c_off[gi], (const double*)W.d_counts.p + c_off[gi],
+                             sizeof(double) * (c_off[gi + 1] - c_off[gi]), hipMemcpyDeviceToHost, L.stream);
       }
-      if (!e) e = hipStreamSynchronize(c->stream);
+      if (!e) e = hipStreamSynchronize(L.stream);
       if (e) {
         const ppm_status q = hip_fail(c, e, "serving read-back");
         for (int gi : gis)
@@ -174,97 +227,134 @@ void worker_loop(ppm_ctx* c) {
       }
       publish(gis);
     };
-    {
-      std::lock_guard<std::mutex> lk(c->exec_mu);
-      hipSetDevice(c->device);
-      int e = 0;
-      if ((e = S.d_flux.ensure(sizeof(double) * f_off[n_groups])) ||
-          (e = S.h_flux.ensure(sizeof(double) * f_off[n_groups])) ||
-          (c_off[n_groups] && ((e = S.d_counts.ensure(sizeof(double) * c_off[n_groups])) ||
-                               (e = S.h_counts.ensure(sizeof(double) * c_off[n_groups]))))) {
-        st.assign(n_groups, hip_fail(c, e, "serving buffers"));
-        std::vector<int> all(n_groups);
-        std::iota(all.begin(), all.end(), 0);
-        publish(all);
-      } else {
-        std::vector<BatchGroup> groups(n_groups);
-        stars.assign(n_groups, {});
+    int e = 0;
+    if ((e = W.d_flux.ensure(sizeof(double) * f_off[n_groups])) ||
+        (e = W.h_flux.ensure(sizeof(double) * f_off[n_groups])) ||
+        (c_off[n_groups] && ((e = W.d_counts.ensure(sizeof(double) * c_off[n_groups])) ||
+                             (e = W.h_counts.ensure(sizeof(double) * c_off[n_groups]))))) {
+      st.assign(n_groups, hip_fail(c, e, "serving buffers"));
+      std::vector<int> all(n_groups);
+      std::iota(all.begin(), all.end(), 0);
+      publish(all);
+    } else {
+      std::vector<BatchGroup> groups(n_groups);
+      stars.assign(n_groups, {});
+      for (int gi = 0; gi < n_groups; ++gi) {
+        const Request& r0 = *batch[members[gi][0]];
+        for (int k : members[gi]) stars[gi].push_back(batch[k]->star);
+        BatchGroup& G = groups[gi];
+        G.stars = stars[gi].data();
+        G.n = (int)stars[gi].size();
+        G.g = r0.grid;
+        G.g.energies_keV = nullptr;
+        G.g.n_energy = (int)r0.energies.size();
+        G.energies = r0.energies;
+        phase_points(G.g, G.pts);
+        G.d_flux = (double*)W.d_flux.p + f_off[gi];
+        G.d_counts = r0.n_ch ? (double*)W.d_counts.p + c_off[gi] : nullptr;
+      }
+      // launches: one per class of grids (at most 128 energies or more,
+      // fine tables up to 384 nodes or more: the kernel instance and the
+      // LDS, hence the occupancy, follow the largest grid of a launch),
+      // groups packed in order while the LDS of one workgroup holds them;
+      // a group the batched kernel cannot take (energies not strictly
+      // increasing, or too large alone) runs on the plain path
+      std::vector<std::vector<int>> launches;
+      std::vector<int> plain;
+      for (int cls = 0; cls < 4; ++cls) {
+        std::vector<BatchGroup> cur;
+        std::vector<int> cur_idx;
         for (int gi = 0; gi < n_groups; ++gi) {
-          const Request& r0 = *batch[members[gi][0]];
-          for (int k : members[gi]) stars[gi].push_back(batch[k]->star);
-          BatchGroup& G = groups[gi];
-          G.stars = stars[gi].data();
-          G.n = (int)stars[gi].size();
-          G.g = r0.grid;
-          G.g.energies_keV = nullptr;
-          G.g.n_energy = (int)r0.energies.size();
-          G.energies = r0.energies;
-          phase_points(G.g, G.pts);
-          G.d_flux = (double*)S.d_flux.p + f_off[gi];
-          G.d_counts = r0.n_ch ? (double*)S.d_counts.p + c_off[gi] : nullptr;
-        }
-        // launches: one per class of grids (at most 128 energies or more,
-        // fine tables up to 384 nodes or more: the kernel instance and the
-        // LDS, hence the occupancy, follow the largest grid of a launch),
-        // groups packed in order while the LDS of one workgroup holds them;
-        // a group the batched kernel cannot take (energies not strictly
-        // increasing, or too large alone) runs on the plain path
-        std::vector<std::vector<int>> launches;
-        std::vector<int> plain;
-        for (int cls = 0; cls < 4; ++cls) {
-          std::vector<BatchGroup> cur;
-          std::vector<int> cur_idx;
-          for (int gi = 0; gi < n_groups; ++gi) {
-            const BatchGroup& G = groups[gi];
-            if (((G.energies.size() > 128) | (G.g.n_fine > 384) << 1) != cls) continue;
-            bool ascending = true;
-            for (size_t q = 1; q < G.energies.size(); ++q) ascending = ascending && G.energies[q] > G.energies[q - 1];
-            if (!ascending || !batch_groups_fit(c, {G})) {
-              plain.push_back(gi);
-              continue;
-            }
-            cur.push_back(G);
-            if (!batch_groups_fit(c, cur)) {
-              cur.pop_back();
-              launches.push_back(cur_idx);
-              cur.assign(1, G);
-              cur_idx.clear();
-            }
-            cur_idx.push_back(gi);
+          const BatchGroup& G = groups[gi];
+          if (((G.energies.size() > 128) | (G.g.n_fine > 384) << 1) != cls) continue;
+          bool ascending = true;
+          for (size_t q = 1; q < G.energies.size(); ++q) ascending = ascending && G.energies[q] > G.energies[q - 1];
+          if (!ascending || !batch_groups_fit(c, {G})) {
+            plain.push_back(gi);
+            continue;
           }
-          if (!cur_idx.empty()) launches.push_back(cur_idx);
-        }
-        // cheapest launch first, each launch's requests completed as soon as
-        // it ends (work ~ stars x pixels x phase points x energies)
-        auto work = [&](const std::vector<int>& L) {
-          double w = 0;
-          for (int gi : L) {
-            const BatchGroup& G = groups[gi];
-            w += (double)G.n * G.g.n_side * G.g.n_side * (double)G.pts.size() * (double)G.energies.size();
+          cur.push_back(G);
+          if (!batch_groups_fit(c, cur)) {
+            cur.pop_back();
+            launches.push_back(cur_idx);
+            cur.assign(1, G);
+            cur_idx.clear();
           }
-          return w;
-        };
-        std::stable_sort(launches.begin(), launches.end(),
-                         [&](const std::vector<int>& a, const std::vector<int>& b) { return work(a) < work(b); });
-        for (const std::vector<int>& L : launches) {
-          std::vector<BatchGroup> lg;
-          for (int gi : L) lg.push_back(groups[gi]);
-          ppm_stats stats{};
-          const ppm_status q = eval_batched_groups(c, lg, &stats);
-          for (int gi : L) st[gi] = q;
-          finish(L);
+          cur_idx.push_back(gi);
         }
-        for (int gi : plain) {
-          ppm_grid g = groups[gi].g;
-          g.energies_keV = groups[gi].energies.data();
-          g.method = PPM_METHOD_PLAIN;
-          ppm_stats stats{};
-          st[gi] = eval_any(c, groups[gi].stars, groups[gi].n, &g, groups[gi].d_flux, groups[gi].d_counts, &stats);
-          finish({gi});
-        }
+        if (!cur_idx.empty()) launches.push_back(cur_idx);
+      }
+      // cheapest launch first, each launch's requests completed as soon as
+      // it ends
+      auto work = [&](const std::vector<int>& Lg) {
+        double wsum = 0;
+        for (int gi : Lg)
+          for (int k : members[gi]) wsum += batch[k]->work;
+        return wsum;
+      };
+      std::stable_sort(launches.begin(), launches.end(),
+                       [&](const std::vector<int>& a, const std::vector<int>& b) { return work(a) < work(b); });
+      for (const std::vector<int>& Lg : launches) {
+        std::vector<BatchGroup> lg;
+        for (int gi : Lg) lg.push_back(groups[gi]);
+        ppm_stats stats{};
+        const ppm_status q = eval_batched_groups(c, L, lg, &stats);
+        for (int gi : Lg) st[gi] = q;
+        finish(Lg);
+      }
+      for (int gi : plain) {
+        ppm_grid g = groups[gi].g;
+        g.energies_keV = groups[gi].energies.data();
+        g.method = PPM_METHOD_PLAIN;
+        ppm_stats stats{};
+        st[gi] = eval_any(c, L, groups[gi].stars, groups[gi].n, &g, groups[gi].d_flux, groups[gi].d_counts, &stats);
+        finish({gi});
       }
     }
+    {
+      size_t fixed = 0;
+      const size_t held = lane_device_bytes(L, &fixed) + W.d_flux.cap + W.d_counts.cap;
+      std::lock_guard<std::mutex> lk(S.mu);
+      W.busy = false;
+      L.wg_max = 0;
+      W.held = held;
+      W.held_fixed = fixed;
+    }
+    S.cv_work.notify_all();  // requests held back for this lane may go elsewhere now
   }
+}
+
+// Work and device memory of a request: evaluations in the reference
+// iteration space (spot solid angle x pixels x phase points x energies), the
+// outputs (device and its pinned host copy count once) and ring tables of its
+// candidate rings, and its accumulators at one work item per spot and energy
+// chunk (the least a batched launch allocates for it; more items only
+// spread over the lane's fixed accumulator cap).
+void request_cost(const ppm_ctx* c, Request& r) {
+  const ppm_grid& g = r.grid;
+  const size_t n_e = r.energies.size();
+  const size_t n_pts = (size_t)g.n_phase * (g.sampling == PPM_SAMPLE_GL ? g.n_gl : 1);
+  const size_t nf1 = (size_t)g.n_fine + 1;
+  size_t n_rings = 0;
+  double area = 0;
+  std::vector<int> rings;
+  for (int s = 0; s < r.star.n_spots; ++s) {
+    SpotParams P;
+    spot_params(c, r.star, r.star.spot[s], g.n_side, P);
+    candidate_rings(P, r.star.spot[s], g.n_side, rings);
+    n_rings += rings.size();
+    area += P.point_source ? 1.0 / (12.0 * g.n_side * g.n_side) : 0.5 * (1.0 - std::cos(r.star.spot[s].angrad1));
+  }
+  r.work = area * 12.0 * g.n_side * g.n_side * (double)n_pts * (double)n_e;
+  const size_t out = sizeof(double) * ((size_t)g.n_phase * n_e + (size_t)r.n_ch * g.n_phase);
+  size_t tables = n_rings * (nf1 * (sizeof(double) + 3 * sizeof(float)) + sizeof(RingDesc) + sizeof(RingTask)) +
+                  sizeof(SpotParams) * r.star.n_spots;
+  bool ascending = true;
+  for (size_t q = 1; q < n_e; ++q) ascending = ascending && r.energies[q] > r.energies[q - 1];
+  if (!ascending)  // the plain path's per-ring partials
+    tables += n_rings * (size_t)g.n_phase * n_e * sizeof(float) + 64 * (size_t)g.n_phase * n_e * sizeof(double);
+  r.bytes = out + tables;
+  r.acc_bytes = ascending ? sizeof(double) * (size_t)r.star.n_spots * batch_acc_copies() * g.n_phase * n_e : 0;
 }
 
 }  // namespace
@@ -272,6 +362,10 @@ void worker_loop(ppm_ctx* c) {
 void serve_init(ppm_ctx* c) {
   c->serve = new ServeState;
   if (c->budget) c->serve->budget = c->budget;
+  // the budget splits evenly between the two workers; each worker's share
+  // caps its launches' accumulators and, separately, the outputs and ring
+  // tables of the requests of one launch (half the share each)
+  c->serve->lane_share = std::min(kAccCapBytes, c->serve->budget / (2 * kWorkers));
 }
 
 void serve_shutdown(ppm_ctx* c) {
@@ -282,11 +376,14 @@ void serve_shutdown(ppm_ctx* c) {
     S->stop = true;
   }
   S->cv_work.notify_all();
-  if (S->worker.joinable()) S->worker.join();
-  S->d_flux.release();
-  S->d_counts.release();
-  S->h_flux.release();
-  S->h_counts.release();
+  for (ServeWorker& W : S->workers) {
+    if (W.thread.joinable()) W.thread.join();
+    W.lane.release();
+    W.d_flux.release();
+    W.d_counts.release();
+    W.h_flux.release();
+    W.h_counts.release();
+  }
   delete S;
   c->serve = nullptr;
 }
@@ -308,24 +405,37 @@ ppm_status ppm_submit(ppm_ctx* c, const ppm_star* star, const ppm_grid* grid, ui
   r->grid.energies_keV = nullptr;  // the copy in r->energies is used
   r->grid.n_energy = (int)r->energies.size();
   r->n_ch = grid->response >= 0 ? c->resp[grid->response].n_ch : 0;
-  // device memory of the request: its outputs, the batched accumulators
-  // (two FP64 copies per item, a few items per star) and ring tables
-  const size_t flux = sizeof(double) * (size_t)grid->n_phase * r->energies.size();
-  r->bytes = flux * 9 + sizeof(double) * (size_t)r->n_ch * grid->n_phase + ((size_t)1 << 20);
+  request_cost(c, *r);
   ServeState& S = *c->serve;
+  if (r->bytes > S.lane_share || r->acc_bytes > S.lane_share)
+    return fail(c, PPM_ERR_NOMEM,
+                "request needs " + std::to_string(r->bytes) + " bytes of tables and outputs and " +
+                    std::to_string(r->acc_bytes) + " of accumulators; one serving launch may use " +
+                    std::to_string(S.lane_share) + " of each (a quarter of the budget)");
   {
     std::lock_guard<std::mutex> lk(S.mu);
     if ((size_t)S.active >= kMaxQueued) return fail(c, PPM_ERR_BUSY, "serving queue full");
-    if (S.charged + r->bytes > S.budget) return fail(c, PPM_ERR_BUSY, "device-memory budget of the serving queue in use");
-    if (!S.worker.joinable()) S.worker = std::thread(worker_loop, c);
-    S.charged += r->bytes;
+    const size_t need = r->bytes + r->acc_bytes;
+    if (S.charged + need > S.budget) return fail(c, PPM_ERR_BUSY, "device-memory budget of the serving queue in use");
+    if (!S.started) {
+      for (int w = 0; w < kWorkers; ++w) {
+        ServeWorker& W = S.workers[w];
+        const int e = W.lane.init();
+        if (e) return hip_fail(c, e, "serving lane");
+        W.lane.no_graph = c->no_graph;
+        W.lane.acc_cap = S.lane_share;
+      }
+      for (int w = 0; w < kWorkers; ++w) S.workers[w].thread = std::thread(worker_loop, c, w);
+      S.started = true;
+    }
+    S.charged += need;
     ++S.active;
     r->t_submit = Clock::now();
     *ticket = S.next_ticket++;
     S.queue.push_back(*ticket);
     S.reqs.emplace(*ticket, std::move(r));
   }
-  S.cv_work.notify_one();
+  S.cv_work.notify_all();
   return PPM_OK;
 }
 
@@ -368,7 +478,7 @@ ppm_status ppm_cancel(ppm_ctx* c, uint64_t ticket) {
     Request& r = *it->second;
     if (r.state == ReqState::kQueued) {
       S.queue.erase(std::find(S.queue.begin(), S.queue.end(), ticket));
-      S.charged -= r.bytes;
+      S.charged -= r.bytes + r.acc_bytes;
       --S.active;
       r.state = ReqState::kCancelled;
     } else if (r.state == ReqState::kRunning) {
@@ -378,6 +488,24 @@ ppm_status ppm_cancel(ppm_ctx* c, uint64_t ticket) {
     }
   }
   S.cv_done.notify_all();
+  return PPM_OK;
+}
+
+ppm_status ppm_serve_stats_get(ppm_ctx* c, ppm_serve_stats* out) {
+  if (!c || !c->serve || !out) return PPM_ERR_ARG;
+  ServeState& S = *c->serve;
+  std::lock_guard<std::mutex> lk(S.mu);
+  *out = ppm_serve_stats{};
+  out->queued = (int64_t)S.queue.size();
+  out->running = S.active - out->queued;
+  for (int w = 0; w < kWorkers; ++w) {
+    out->launches[w] = S.workers[w].launches;
+    out->device_bytes += S.workers[w].held;
+    out->fixed_bytes += S.workers[w].held_fixed;
+  }
+  out->overlapped = S.workers[kExpress].overlapped;
+  out->charged_bytes = S.charged;
+  out->budget_bytes = S.budget;
   return PPM_OK;
 }
 

@@ -28,7 +28,9 @@ def runtime():
         h.hipMemset.argtypes = [c_void_p, c_int, c_size_t]
         h.hipDeviceSynchronize.argtypes = []
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(c_int)]
-        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipGetDeviceCount"):
+        h.hipMemGetInfo.argtypes = [ctypes.POINTER(c_size_t), ctypes.POINTER(c_size_t)]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipGetDeviceCount",
+                  "hipMemGetInfo"):
             getattr(h, f).restype = c_int
         _hip = h
     return _hip
@@ -47,6 +49,13 @@ def device_count():
 
 def synchronize():
     _check(runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    free, total = c_size_t(0), c_size_t(0)
+    _check(runtime().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
 
 
 def copy_device(dst, src, nbytes):

@@ -69,6 +69,8 @@ def load():
     L.ppm_poll.argtypes = [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
     L.ppm_cancel.restype = c_int
     L.ppm_cancel.argtypes = [c_void_p, c_uint64]
+    L.ppm_serve_stats_get.restype = c_int
+    L.ppm_serve_stats_get.argtypes = [c_void_p, POINTER(abi.ServeStats)]
     L.ppm_drain.restype = c_int
     L.ppm_drain.argtypes = [c_void_p]
     L.ppm_eval_fit.restype = c_int
@@ -267,6 +269,11 @@ class Context:
 
     def cancel(self, ticket):
         return load().ppm_cancel(self._h, ticket)
+
+    def serve_stats(self):
+        out = abi.ServeStats()
+        self._check(load().ppm_serve_stats_get(self._h, byref(out)))
+        return out
 
     def drain(self):
         self._check(load().ppm_drain(self._h))

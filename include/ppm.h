@@ -238,6 +238,22 @@ ppm_status ppm_cancel(ppm_ctx* ctx, uint64_t ticket);
 /* Block until every admitted request is finished or cancelled. */
 ppm_status ppm_drain(ppm_ctx* ctx);
 
+/* State of the serving loop.  No counterpart in the reference (it has no
+ * serving layer).  device_bytes counts the device buffers the serving lanes
+ * hold (grow-only, reused by later launches); the admission budget bounds
+ * them apart from each lane's segment-total scratch (fixed_bytes: up to one
+ * block per resident workgroup of the device, about 9 MB). */
+typedef struct ppm_serve_stats {
+  int64_t queued, running;   /* requests waiting, and in a launch              */
+  int64_t launches[2];       /* launch sequences of the express and bulk lanes */
+  int64_t overlapped;        /* express launches started while a bulk launch ran */
+  size_t charged_bytes;      /* admission charges of queued and running requests */
+  size_t budget_bytes;       /* the context's hbm_budget_bytes                  */
+  size_t device_bytes;       /* device memory held by the serving lanes          */
+  size_t fixed_bytes;        /* of which segment-total scratch                   */
+} ppm_serve_stats;
+ppm_status ppm_serve_stats_get(ppm_ctx* ctx, ppm_serve_stats* out);
+
 /* Fit of one star's folded counts m[c][p] to data d[c][p]. */
 typedef struct ppm_fit {
   double chi2;        /* sum (d - m)^2 / m with model and data scaled so that the model sums

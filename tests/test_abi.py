@@ -33,10 +33,11 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include "ppm.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ppm_spot), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ppm_spot), '
                    'sizeof(ppm_star), sizeof(ppm_grid), sizeof(ppm_nsx_config), sizeof(ppm_open_opts), '
                    'sizeof(ppm_stats), offsetof(ppm_star, spot), offsetof(ppm_grid, energies_keV), '
-                   'sizeof(ppm_launch_shape), sizeof(ppm_tune_result), offsetof(ppm_tune_result, n_tried));'
+                   'sizeof(ppm_launch_shape), sizeof(ppm_tune_result), offsetof(ppm_tune_result, n_tried), '
+                   'sizeof(ppm_serve_stats), offsetof(ppm_serve_stats, device_bytes));'
                    'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
@@ -44,7 +45,8 @@ def test_struct_layouts_match_c(tmp_path):
     want = [ctypes.sizeof(abi.Spot), ctypes.sizeof(abi.Star), ctypes.sizeof(abi.Grid),
             ctypes.sizeof(abi.NsxConfig), ctypes.sizeof(abi.OpenOpts), ctypes.sizeof(abi.Stats),
             abi.Star.spot.offset, abi.Grid.energies_keV.offset, ctypes.sizeof(abi.LaunchShape),
-            ctypes.sizeof(abi.TuneResult), abi.TuneResult.n_tried.offset]
+            ctypes.sizeof(abi.TuneResult), abi.TuneResult.n_tried.offset, ctypes.sizeof(abi.ServeStats),
+            abi.ServeStats.device_bytes.offset]
     assert got == want
 
 

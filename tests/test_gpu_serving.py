@@ -104,8 +104,12 @@ def test_cancel_and_unknown_ticket(ctx):
 
 
 def test_budget_admission(data_root, small_nsx_cfg):
+    """Requests are admitted against the budget (outputs, ring tables and
+    their least accumulators); the device memory the serving lanes hold
+    after a stream of requests stays within it."""
+    budget = 2 << 20
     with Context(data_root, lensing="wide", nsx_path=fixtures.nsx_path(data_root, small=True),
-                 responses=("std",), nsx_config=small_nsx_cfg, hbm_budget_bytes=4 << 20) as c:
+                 responses=("std",), nsx_config=small_nsx_cfg, hbm_budget_bytes=budget) as c:
         g = abi.make_grid(32, 16, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
         stars = S.sample_priors(8, seed=2)
         print("case: serving budget", flush=True)
@@ -123,3 +127,58 @@ def test_budget_admission(data_root, small_nsx_cfg):
         # memory is returned when requests finish: admission works again
         t = c.submit(stars[0], g)
         assert wait(c, t, g)[0] == abi.OK
+        for st in stars:  # one by one: every request fits alone
+            assert wait(c, c.submit(st, g), g)[0] == abi.OK
+        ss = c.serve_stats()
+        print("  serving lanes hold %d bytes, %d of them segment scratch (budget %d); launches %d + %d"
+              % (ss.device_bytes, ss.fixed_bytes, budget, ss.launches[0], ss.launches[1]), flush=True)
+        assert ss.budget_bytes == budget and ss.charged_bytes == 0 and ss.queued == ss.running == 0
+        assert 0 < ss.device_bytes - ss.fixed_bytes <= budget
+        # a request whose tables alone exceed the budget's launch share is refused
+        big = abi.make_grid(1024, 64, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
+        with pytest.raises(PpmError) as ei:
+            c.submit(stars[0], big)
+        assert ei.value.status == abi.ERR_NOMEM
+
+
+def test_light_request_overtakes_a_heavy_launch(ctx, data_root, small_nsx_cfg):
+    """A light request submitted while a heavy N_side 2048 request runs on
+    the device starts in the workgroup slots the heavy launch leaves free and
+    finishes first; both results equal their direct evaluations and the
+    light one the oracle."""
+    heavy_g = abi.make_grid(2048, 128, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
+    light_g = abi.make_grid(64, 16, sampling=abi.SAMPLE_GL, n_gl=2, response=1)
+    base = S.sample_priors(3, seed=51)
+    heavy, warm, light = base
+    heavy.spot[0].angrad1 = 1.0   # about 1e12 evaluations: far above the light-work threshold
+    warm.spot[0].angrad1 = 0.6
+    light.spot[0].angrad1 = 0.3
+    print("case: warm-up of both grids", flush=True)
+    for t, g in [(ctx.submit(warm, heavy_g), heavy_g), (ctx.submit(light, light_g), light_g)]:
+        assert wait(ctx, t, g)[0] == abi.OK
+    print("case: a heavy request, then a light one 20 ms later", flush=True)
+    t0 = time.perf_counter()
+    th = ctx.submit(heavy, heavy_g)
+    time.sleep(0.02)
+    t_light = time.perf_counter()
+    tl = ctx.submit(light, light_g)
+    st, fl, cl, lat_l = wait(ctx, tl, light_g)
+    assert st == abi.OK
+    sh, fh_s, ch_s, lat_h = wait(ctx, th, heavy_g)
+    assert sh == abi.OK
+    done_light, done_heavy = t_light + lat_l * 1e-3, t0 + lat_h * 1e-3
+    print("  light done at %.1f ms, heavy at %.1f ms (after the heavy submit)"
+          % ((done_light - t0) * 1e3, (done_heavy - t0) * 1e3), flush=True)
+    assert done_light < done_heavy
+    assert ctx.serve_stats().overlapped >= 1
+    fd, cd, _ = ctx.eval([light], light_g)
+    assert rel_err(fl, fd[0]) <= 1e-5 and rel_err(cl, cd[0]) <= 1e-5
+    fh, ch, _ = ctx.eval([heavy], heavy_g)
+    assert rel_err(fh_s, fh[0]) <= 1e-5 and rel_err(ch_s, ch[0]) <= 1e-5
+    orc = Oracle(data_root, lensing="wide", nsx_path=fixtures.nsx_path(data_root, small=True),
+                 responses=("std", "std128"), nsx_cfg=small_nsx_cfg)
+    try:
+        o, _ = orc.profile(light, light_g, threads=16)
+        assert rel_err(fl, o) <= 1e-4
+    finally:
+        orc.close()
