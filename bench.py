@@ -52,37 +52,140 @@ def grid(method):
     return abi.make_grid(N_SIDE, N_PHASE, sampling=abi.SAMPLE_GL, n_gl=2, response=0, method=method)
 
 
-def cpu_baseline(root, stars, lensing, target_s=20.0):
-    """The FP64 oracle (a C port of the reference CPU path) on a bounded
-    sample of the same workload: the first star(s), all rings and phase bins,
-    a subset of the energies; the per-profile time is scaled by the energy
-    ratio."""
+def host_cores():
+    """CPU threads this job may use: the affinity mask, limited by the cgroup
+    CPU quota when one is set (a container's share of a larger host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            n = min(n, max(1, int(int(q) // int(per))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def reference_os1_cpu(root):
+    """The reference's own CPU program, cpu/os1_cpu.cpp built unmodified from
+    its sources (oracle/_ref/os1_cpu), on its ten hard-coded os1 scenarios
+    (N_side 256, 128 phase points, blackbody at 1 keV), timed by process wall
+    clock (the program has no internal timer), one thread."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "os1_cpu")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        os.makedirs(os.path.join(d, "tables", "lensing"))
+        os.makedirs(os.path.join(d, "output"))
+        os.symlink(os.path.join(root, "tables", "lensing", "std"), os.path.join(d, "tables", "lensing", "std"))
+        t0 = time.perf_counter()
+        r = subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        dt = time.perf_counter() - t0
+    if r.returncode != 0:
+        return None
+    return {"kind": "reference", "s_per_profile": dt / 10, "profiles": 10, "cores": 1,
+            "what": "unmodified cpu/os1_cpu.cpp (reference flags minus -march=native), os1 a-j: N_side 256, "
+                    "128 phase points, blackbody at 1 keV; process wall time / 10"}
+
+
+def cpu_config0(root, threads):
+    """The FP64 oracle on config 0 at full size (cap 0.25 rad at 20 deg,
+    blackbody kT 0.35 keV, N_side 128, 32 bins x 2 GL, 128 energies, fold
+    300 x 128), on 1 thread and on `threads`, the reference's timing
+    boundaries (integral + fold; tables loaded and the grid set up outside)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from gpu_ppm_amd import abi
     from gpu_ppm_amd import scenarios as S
     from oracle_binding import Oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    orc = Oracle(root, lensing="std", responses=("std", "std128"))
+    star = S.cfg0_star()
+    g = abi.make_grid(128, 32, sampling=abi.SAMPLE_GL, n_gl=2, response=1)
+    out = {"config": "cap 0.25 rad at 20 deg, blackbody kT 0.35 keV, N_side 128, 32 bins x 2 GL, 128 E, fold 300x128"}
+    for key, th in (("1_thread", 1), ("all_cores", threads)):
+        best = 1e300
+        for _ in range(3):
+            t0 = time.perf_counter()
+            f, info = orc.profile(star, g, threads=th)
+            orc.fold(f, 1, star.exposure_s / g.n_phase)
+            best = min(best, time.perf_counter() - t0)
+        out[key] = {"threads": th, "s_per_profile": best, "evals_per_s": info.n_eval_ref / best}
+    out["n_eval_ref"] = info.n_eval_ref
+    orc.close()
+    return out
+
+
+def cpu_baseline(root, stars, lensing, batched_flux, target_s=20.0):
+    """The FP64 oracle (a C port of the reference CPU path) on a bounded
+    sample of the same workload: the first star(s), all rings and phase bins,
+    a subset of the energies; the per-profile time is scaled by the energy
+    ratio.  The oracle's columns are compared with the same columns of the
+    batched and the plain device results.  Also: the oracle on config 0 at
+    full size (1 thread and all cores) and the reference's own os1_cpu."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from gpu_ppm_amd import abi, hip
+    from gpu_ppm_amd import scenarios as S
+    from gpu_ppm_amd.lib import Context
+    from oracle_binding import Oracle
+    threads = host_cores()
     orc = Oracle(root, lensing=lensing, nsx_path=os.path.join(root, "ext", "model_data", "nsx_H_v200804.out"),
                  responses=("std",))
     energies = S.geomspace(0.2, 12.0, N_E)
 
+    def columns(n_e):
+        return [int(round(k * (N_E - 1) / max(n_e - 1, 1))) for k in range(n_e)]
+
     def run(n_e, sts):
-        idx = [int(round(k * (N_E - 1) / max(n_e - 1, 1))) for k in range(n_e)]
+        idx = columns(n_e)
         g = abi.make_grid(N_SIDE, N_PHASE, sampling=abi.SAMPLE_GL, n_gl=2, energies=[energies[i] for i in idx])
         t0 = time.perf_counter()
-        for st in sts:
-            orc.profile(st, g, threads=threads)
-        return time.perf_counter() - t0
+        out = [orc.profile(st, g, threads=threads)[0] for st in sts]
+        return time.perf_counter() - t0, out
 
     sample = stars[:2]
-    t2 = run(2, sample)
+    t2, _ = run(2, sample)
     n_e = max(2, min(N_E, int(2 * target_s / max(t2, 1e-3))))
-    t = run(n_e, sample)
+    t, ref = run(n_e, sample)
     orc.close()
+    idx = columns(n_e)
+    # the same columns of the device results: the batched launch of the timed
+    # workload, and the plain path on the same stars
+    gp = grid(abi.METHOD_PLAIN)
+    plain_ctx = Context(root, lensing=lensing, responses=("std",))
+    n_ch = plain_ctx.response_info(0)[0]
+    d_f = hip.DeviceArray((1, N_PHASE, N_E))
+    d_c = hip.DeviceArray((1, n_ch, N_PHASE))
+    err = {"batched_flux": 0.0, "plain_flux": 0.0, "stars": len(sample), "energy_columns": idx}
+    for k, st in enumerate(sample):
+        if batched_flux is not None:
+            err["batched_flux"] = max(err["batched_flux"], rel_err(batched_flux[k][:, idx], ref[k]))
+        plain_ctx.eval_device([st], gp, d_f.ptr, d_c.ptr)
+        hip.synchronize()
+        err["plain_flux"] = max(err["plain_flux"], rel_err(d_f.numpy()[0][:, idx], ref[k]))
+    plain_ctx.close()
+    log("bench: device vs FP64 oracle columns %s" % err)
     per_profile = t * N_E / n_e / len(sample)
-    return {"value": 1.0 / per_profile, "unit": "profiles/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle on %d of the workload's stars, all rings and bins, %d of %d energies, "
-                      "%.1f s on %d threads; scaled by %d/%d" % (len(sample), n_e, N_E, t, threads, N_E, n_e)}
+    out = {"value": 1.0 / per_profile, "unit": "profiles/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": "FP64 oracle on %d of the workload's stars, all rings and bins, %d of %d energies, "
+                     "%.1f s on %d threads; scaled by %d/%d" % (len(sample), n_e, N_E, t, threads, N_E, n_e),
+           "max_rel_err_vs_fp64_oracle": err}
+    log("bench: CPU config 0")
+    out["config0"] = cpu_config0(root, threads)
+    ref_run = reference_os1_cpu(root)
+    if ref_run:
+        out["reference_os1_cpu"] = ref_run
+    return out
 
 
 def rel_err(x, ref, floor=1e-6):
@@ -149,6 +252,18 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
     hip.synchronize()
     t_plain = time.perf_counter() - t0
     log("bench: plain %d stars in %.2f s" % (len(sub), t_plain))
+    # the batched path (phase-shift dedupe) on one profile per launch, the
+    # same stars: separates the dedupe's gain from the batching's
+    gs = grid(abi.METHOD_BATCHED)
+    ctx.eval_device([sub[0]], gs, d_f1.ptr, d_c1.ptr)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    single_int_ms = 0.0
+    for s in sub:
+        single_int_ms += ctx.eval_device([s], gs, d_f1.ptr, d_c1.ptr).integral_ms
+    hip.synchronize()
+    t_single = time.perf_counter() - t0
+    log("bench: dedupe, one profile per launch: %d stars in %.2f s" % (len(sub), t_single))
     # outside the timed regions: the batched results of the direct launch
     # against the plain path on every 8th timed star
     fb, cb = d_flux.numpy(), d_cnt.numpy()
@@ -167,7 +282,9 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
                 kernel="k_integral_batched", kernel_ms=k.integral_ms, n_eval=k.n_eval_done,
                 n_eval_ref=k.n_eval_ref, n_active=k.n_active,
                 plain_profiles_per_s=len(sub) / t_plain, plain_stars=len(sub),
-                plain_evals_per_s=ref_evals / (plain_int_ms * 1e-3),
+                plain_evals_per_s=ref_evals / (plain_int_ms * 1e-3), plain_integral_ms=plain_int_ms / len(sub),
+                single_profiles_per_s=len(sub) / t_single, single_integral_ms=single_int_ms / len(sub),
+                batched_flux=fb[:2].copy(),
                 graph_ms=sum(s.total_ms for s in st) / steps, setup_reduce_fold_ms=k.total_ms - k.integral_ms,
                 launch_shape=shape, max_rel_err=err)
 
@@ -183,7 +300,8 @@ def bench_serving(root, duration_s, burst, seed=7):
     once; the arrival rate is then 80 % of it for `duration_s` seconds.
     Latency is submit -> result on the host, reported by ppm_poll.  Every
     100th request of the timed stream is audited afterwards against a direct
-    plain-path evaluation of the same star and grid."""
+    plain-path evaluation of the same star and grid, and against the FP64
+    oracle on one energy column."""
     import numpy as np
     from gpu_ppm_amd import abi
     from gpu_ppm_amd import scenarios as S
@@ -232,7 +350,12 @@ def bench_serving(root, duration_s, burst, seed=7):
     reqs = draw(n_req)
     log("bench: serving %d requests at %.1f/s (capacity %.1f/s)" % (n_req, rate, capacity))
     audit = {}
+    s1 = ctx.serve_stats()
     dt, lat = run(reqs, arrivals, audit)
+    s2 = ctx.serve_stats()
+    lanes = {"express_launches": s2.launches[0] - s1.launches[0], "bulk_launches": s2.launches[1] - s1.launches[1],
+             "express_launches_overlapping_bulk": s2.overlapped - s1.overlapped,
+             "device_bytes_held": s2.device_bytes}
     err = {"flux": 0.0, "counts": 0.0, "requests_audited": len(audit)}
     for k, (f, q) in audit.items():
         star, c = reqs[k]
@@ -241,6 +364,28 @@ def bench_serving(root, duration_s, burst, seed=7):
         f0, q0, _ = ctx.eval([star], g)
         err["flux"] = max(err["flux"], rel_err(f, f0[0]))
         err["counts"] = max(err["counts"], rel_err(q, q0[0]))
+    # the same audited requests against the FP64 oracle on one energy column
+    # each (the middle of the grid; every column of a profile is independent),
+    # within about a minute of CPU time
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_binding import Oracle
+    orc = Oracle(root, lensing="wide", nsx_path=os.path.join(root, "ext", "model_data", "nsx_H_v200804.out"),
+                 responses=("std", "std128"))
+    err_o = {"flux": 0.0, "requests_audited": 0, "energy_column": "middle of each grid", "oracle_s": 0.0}
+    t_o = time.perf_counter()
+    for k, (f, q) in sorted(audit.items()):
+        if time.perf_counter() - t_o > 60:
+            break
+        star, c = reqs[k]
+        e_all = orc.response(grids[c].response)[0]
+        col = len(e_all) // 2
+        gc = abi.make_grid(c[0], c[1], sampling=abi.SAMPLE_GL, n_gl=2, energies=[float(e_all[col])])
+        ref, _ = orc.profile(star, gc, threads=host_cores())
+        err_o["flux"] = max(err_o["flux"], rel_err(f[:, [col]], ref))
+        err_o["requests_audited"] += 1
+    err_o["oracle_s"] = time.perf_counter() - t_o
+    orc.close()
+    log("bench: served vs plain %s, vs FP64 oracle %s" % (err, err_o))
     ref_evals = 0
     for star, c in reqs:
         ref_evals += ctx.count_active(star, grids[c]) * c[1] * 2 * c[2]
@@ -249,6 +394,7 @@ def bench_serving(root, duration_s, burst, seed=7):
             "completed_per_s": n_req / dt, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
             "evals_per_s_ref": ref_evals / dt, "audit_max_rel_err_vs_plain": err,
+            "audit_max_rel_err_vs_fp64_oracle": err_o, "lanes": lanes,
             "workload": "prior random walk (2%% steps), N_side {256,1024,2048} x bins {32,64,128} x 2 GL "
                         "x E {128,360}, open-loop Poisson arrivals at 80%% of burst capacity for %.0f s"
                         % duration_s}
@@ -349,7 +495,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": cfg,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": committed_traffic(r["kernel"]) if args.workload == "batched" and args.stars == 256
                      else None,
@@ -358,9 +504,17 @@ def main():
                      "units": unit_note, "ref_equivalent_tflops": ref_eq},
     }
     if args.workload == "batched":
+        plain_tf = r["plain_evals_per_s"] * FLOP_PER_EVAL / 1e12
         out["plain"] = {"profiles_per_s": r["plain_profiles_per_s"], "stars_timed": r["plain_stars"],
-                        "evals_per_s_ref": r["plain_evals_per_s"]}
+                        "evals_per_s_ref": r["plain_evals_per_s"], "integral_ms_per_profile": r["plain_integral_ms"],
+                        "kernel": "k_integral_plain", "achieved_tflops": plain_tf,
+                        "frac": plain_tf / FP32_PEAK_TFLOPS}
+        out["dedupe_single"] = {"profiles_per_s": r["single_profiles_per_s"],
+                                "integral_ms_per_profile": r["single_integral_ms"],
+                                "what": "the batched path (phase-shift dedupe) with one star per launch, same stars"}
         out["batched_vs_plain"] = r["value"] / r["plain_profiles_per_s"]
+        out["batching_only_factor"] = r["value"] / r["single_profiles_per_s"]
+        out["dedupe_single_vs_plain"] = r["single_profiles_per_s"] / r["plain_profiles_per_s"]
         out["batched_vs_plain_max_rel_err"] = r["max_rel_err"]
         out["batched"] = {"graph_ms_per_step": r["graph_ms"], "integral_ms": r["kernel_ms"],
                           "setup_reduce_fold_ms": r["setup_reduce_fold_ms"], "n_eval_ref": r["n_eval_ref"],
@@ -369,7 +523,7 @@ def main():
     if args.workload == "batched" and args.serve_seconds > 0:
         out["serving"] = bench_serving(root, args.serve_seconds, args.serve_burst)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(root, r["stars"], r["lensing"])
+        out["cpu_baseline"] = cpu_baseline(root, r["stars"], r["lensing"], r.get("batched_flux"))
     print(json.dumps(out), flush=True)
 
 

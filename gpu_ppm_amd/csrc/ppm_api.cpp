@@ -603,7 +603,7 @@ ppm_status ppm_open(const ppm_open_opts* opts, ppm_ctx** out) {
   if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= o.device || o.device < 0) return PPM_ERR_DEVICE;
   auto* c = new ppm_ctx();
   c->device = o.device;
-  c->budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : ((size_t)16 << 30);
+  c->budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : ((size_t)48 << 30);
   int e;
   if ((e = hipSetDevice(o.device)) || (e = c->main.init()) ||
       (e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, o.device))) {

@@ -444,9 +444,6 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
     if (it != BS->shapes.end()) shape = it->second;
   }
   if (shape.wg_per_cu > 0) wg_per_cu = std::min(wg_per_cu, shape.wg_per_cu);
-  // workgroup slots left to the other lanes of the context
-  if (L.wg_leave > 0) wg_per_cu = std::max(1, wg_per_cu - L.wg_leave);
-  if (L.wg_max > 0) wg_per_cu = std::min(wg_per_cu, L.wg_max);
   const int slots = wg_per_cu * c->n_cu;
 
   // ---- items: each spot's rings split into chunks of similar cost, times
@@ -550,7 +547,8 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
       (e = W.evals.ensure(sizeof(unsigned long long))) ||
       (e = W.summary.ensure(sizeof(unsigned long long) * (4 * n_groups + 1))) ||
       (e = LB->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
-      (e = LB->d_seg.ensure(sizeof(double) * batch_seg_doubles() * std::max(slots, 1))) ||
+      (e = LB->d_seg.ensure(sizeof(double) * batch_seg_doubles() *
+                            std::max(L.per_item ? n_items : std::min(slots, n_items), 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
     return hip_fail(c, e, "device allocation");
   std::vector<BatchGrid> bg(n_groups);
@@ -621,7 +619,8 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   B.epl = epl;
   B.atmosphere = atm;
   B.emode = emode;
-  const int n_wg = std::max(1, std::min(slots, n_items));
+  B.persistent = !L.per_item;
+  const int n_wg = L.per_item ? std::max(1, n_items) : std::max(1, std::min(slots, n_items));
 
   // ---- the launch sequence, captured once per configuration and replayed
   std::vector<unsigned char> key;

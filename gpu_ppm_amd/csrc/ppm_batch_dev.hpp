@@ -64,6 +64,8 @@ struct BatchLaunch {
   int n_fine_max;          // LDS layout: the largest grid of the launch
   int nsx_rows, cov_words, ev_cap, atmosphere;
   int epl;                 // energies per lane: 6, or 2 when every grid has <= 128 energies
+  int persistent;          // 1: resident workgroups take items from the atomic queue;
+                           // 0: one workgroup per item (workgroup b takes queue position b)
   int emode;               // NsxEnergyMode shared by the launch's stars
   unsigned long long* rows_flag;  // bit 1 set when a ring needs more NSX rows than staged
 };

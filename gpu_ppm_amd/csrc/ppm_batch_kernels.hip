@@ -1,7 +1,9 @@
 // Batched integral for gfx950: many stars in one persistent launch.
 //
 // Work queue: items = (star, spot, contiguous range of ring tasks), taken
-// largest-first by persistent workgroups through an atomic counter.
+// largest-first, either by persistent workgroups through an atomic counter
+// or one item per workgroup in dispatch order (workgroups then retire every
+// item, and the device can interleave another stream's launch among them).
 //
 // Phase-shift dedupe.  On a ring of n pixels, patch j seen at phase point p
 // has rotational phase x = frac(p + (j - s/2)/n).  Writing p n = m + delta,
@@ -159,9 +161,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   const double inv_h = 1.0 / h;
   unsigned int n_done = 0;  // evaluations by this lane
 
-  for (;;) {
+  for (int round = 0;; ++round) {
     __syncthreads();
-    if (tid == 0) s_item = atomicAdd(B.queue, 1);
+    if (tid == 0) s_item = B.persistent ? atomicAdd(B.queue, 1) : (round == 0 ? (int)blockIdx.x : B.n_items);
     __syncthreads();
     const int q_idx = s_item;
     if (q_idx >= B.n_items) break;

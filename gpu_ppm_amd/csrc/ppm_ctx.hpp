@@ -87,9 +87,10 @@ struct Lane {
   bool no_graph = false;                     // launch the batched sequence directly
   const ppm_launch_shape* forced = nullptr;  // shape under test by ppm_tune
   size_t acc_cap = kAccCapBytes;             // accumulator bytes of one batched launch
-  int wg_leave = 0;  // resident workgroups per CU the batched launches leave to other lanes
-  int wg_max = 0;    // resident workgroups per CU of the next batched launch at most (0: no cap)
   int occ_wg_max = 0;  // occupancy limit of the last batched launch
+  // batched launches with one workgroup per work item instead of persistent
+  // workgroups (a concurrent launch on another lane gets slots as they retire)
+  bool per_item = false;
   int init();
   void release();
 };

@@ -5,12 +5,13 @@
 // estimates its work and device memory, charges the memory against the
 // context's budget and queues it.  Two workers, each with its own lane
 // (stream and buffers), take queued requests whenever they are free:
-//   - the bulk worker takes any request; its persistent launches leave one
-//     resident workgroup per CU free;
+//   - the bulk worker takes every queued request (the largest batches);
 //   - the express worker takes light requests (estimated work below
-//     kLightWork); while the bulk worker has a launch in flight its launches
-//     run in the free workgroup per CU, concurrently with that launch, so a
-//     light request that arrives during a heavy launch does not wait for it.
+//     kLightWork) while the bulk worker has a launch in flight.
+// Serving launches run one workgroup per work item (not persistent
+// workgroups), so workgroups retire every few milliseconds and the device
+// dispatches an express launch's workgroups among a running bulk launch's:
+// a light request that arrives during a heavy launch does not wait for it.
 // A worker takes every queued request it may serve of the oldest such
 // request's launch class (atmosphere and, for NSX, the interpolant's
 // energy-axis evaluation), up to its lane's memory share, groups them by
@@ -38,7 +39,6 @@ using Clock = std::chrono::steady_clock;
 
 constexpr int kMaxBatch = 1024;          // requests per launch sequence
 constexpr size_t kMaxQueued = 1 << 16;   // queued + running requests
-constexpr size_t kDefaultBudget = (size_t)16 << 30;
 constexpr double kLightWork = 1e11;      // reference-space evaluations of a light request
 enum { kExpress = 0, kBulk = 1, kWorkers = 2 };
 
@@ -85,7 +85,7 @@ struct ServeState {
   std::deque<uint64_t> queue;  // queued tickets, arrival order
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_ticket = 1;
-  size_t budget = kDefaultBudget, charged = 0;
+  size_t budget = 0, charged = 0;  // the context's hbm_budget_bytes
   size_t lane_share = 0;  // per worker: accumulator cap, and request bytes of one launch at most
   int active = 0;         // queued + running
   bool stop = false, started = false;
@@ -96,9 +96,8 @@ namespace {
 
 // Whether worker w may take request r now (S.mu held).
 bool eligible(const ServeState& S, int w, const Request& r) {
-  const bool light = r.work < kLightWork;
-  if (w == kExpress) return light;
-  return !light || S.workers[kExpress].busy;  // light requests go to the bulk lane when express is busy
+  if (w == kBulk) return true;
+  return S.workers[kBulk].busy && r.work < kLightWork;
 }
 
 void worker_loop(ppm_ctx* c, int w) {
@@ -142,15 +141,7 @@ void worker_loop(ppm_ctx* c, int w) {
         }
       }
       W.busy = true;
-      // workgroup slots: bulk launches always leave one per CU free; an
-      // express launch takes that one only while a bulk launch runs, else
-      // the whole device
-      if (w == kBulk) {
-        L.wg_leave = 1;
-      } else {
-        L.wg_max = S.workers[kBulk].busy ? 1 : 0;
-        W.overlapped += S.workers[kBulk].busy;
-      }
+      if (w == kExpress) ++W.overlapped;  // express launches start only while a bulk launch runs
       ++W.launches;
     }
     const int n = (int)batch.size();
@@ -316,7 +307,6 @@ void worker_loop(ppm_ctx* c, int w) {
       const size_t held = lane_device_bytes(L, &fixed) + W.d_flux.cap + W.d_counts.cap;
       std::lock_guard<std::mutex> lk(S.mu);
       W.busy = false;
-      L.wg_max = 0;
       W.held = held;
       W.held_fixed = fixed;
     }
@@ -424,6 +414,7 @@ ppm_status ppm_submit(ppm_ctx* c, const ppm_star* star, const ppm_grid* grid, ui
         if (e) return hip_fail(c, e, "serving lane");
         W.lane.no_graph = c->no_graph;
         W.lane.acc_cap = S.lane_share;
+        W.lane.per_item = true;
       }
       for (int w = 0; w < kWorkers; ++w) S.workers[w].thread = std::thread(worker_loop, c, w);
       S.started = true;

@@ -145,7 +145,8 @@ typedef struct ppm_open_opts {
                                        13.7-14.7 (cpu/nsx.hpp:39-51)         */
   int device;                    /* HIP device ordinal                               */
   int reserved;
-  size_t hbm_budget_bytes;       /* serving admission budget; 0 -> 16 GiB            */
+  size_t hbm_budget_bytes;       /* device memory of batched launches and of the
+                                    serving queue (admission); 0 -> 48 GiB          */
 } ppm_open_opts;
 
 typedef struct ppm_stats {
