@@ -65,6 +65,20 @@ def ensure_nsx(root, small=False):
     return p
 
 
+def nsx_bb_path(root):
+    return os.path.join(root, "ext", "model_data", "nsx_blackbody.out")
+
+
+def ensure_nsx_bb(root):
+    """NSX-format grid on the reduced 6x5x166x67 shape (nsx_H_v200804 axes)
+    filled with the isotropic Planck intensity alone: logInuT3 =
+    log10(2 k^3 x^3 / (h^2 c^2 (e^x - 1))), x = 10^logEkT."""
+    p = nsx_bb_path(root)
+    if not os.path.exists(p):
+        _synth("nsx_bb", p, *SMALL_NSX_DIMS)
+    return p
+
+
 def ensure_nsx_v171019(root):
     """NSX-format grid on the nsx_H_v171019 axes (35x11x166x67, log g
     13.7-14.7, cpu/nsx.hpp:39-51), seed 1."""

@@ -21,6 +21,12 @@
  *       intensity B_nu(T)/T^3 (1 + a mu) (1 + 0.02 n), n a seeded smooth
  *       noise with |n| <= 1.
  *
+ *   nsx_bb   PATH [nT ng nE nmu]
+ *       The same grid filled with the isotropic Planck intensity alone
+ *       (a = 0, no noise): logInuT3 = log10(2 k^3 x^3 / (h^2 c^2 (e^x - 1))),
+ *       x = E/kT, independent of logT, log g and mu, so that the NSX path
+ *       reproduces the blackbody path up to the energy interpolation.
+ *
  *   response OUTDIR n_ch n_E [seed]
  *       tables/ism_inst/<setting>/{num,E_obs,rsp}.txt (c_ism_inst.py:61-101
  *       formats): n_ch channels linear in 0.2-12 keV, n_E log-spaced E_obs,
@@ -328,12 +334,12 @@ static int cmd_lensing(const char* dir, double u_min, double u_max, int n_u, int
 
 /* ------------------------------------------------------------ NSX grid */
 
-static int cmd_nsx(const char* path, uint64_t seed, int nT, int ng, int nE, int nm, double logg_max) {
+static int cmd_nsx(const char* path, uint64_t seed, int nT, int ng, int nE, int nm, double logg_max, double limb_a,
+                   double noise_amp) {
   const double logT_min = 5.10, logT_max = 6.80, logg_min = 13.7;
   const double logE_min = -1.30, logE_max = 2.00; /* cpu/nsx.hpp:25-37 */
   const double kB = 1.380649e-16, h = 6.62607015e-27, c = 2.99792458e10;
   const double pref = 2.0 * kB * kB * kB / (h * h * c * c);
-  const double limb_a = 0.5;
   sm_state = seed;
   double amp[4], kv[4][4], ph[4], asum = 0;
   for (int m = 0; m < 4; ++m) {
@@ -379,7 +385,7 @@ static int cmd_nsx(const char* path, uint64_t seed, int nT, int ng, int nE, int 
           for (int m = 0; m < 4; ++m)
             noise += amp[m] * sin(kv[m][0] * v[0] + kv[m][1] * v[1] + kv[m][2] * v[2] +
                                   kv[m][3] * v[3] + ph[m]);
-          double it3 = planck * (1.0 + limb_a * mu[im]) * (1.0 + 0.02 * noise);
+          double it3 = planck * (1.0 + limb_a * mu[im]) * (1.0 + noise_amp * noise);
           fprintf(f, "%.10g %.10g %.12g %.10g %.10g\n", logE, mu[im], log10(it3), logT, logg);
         }
       }
@@ -460,7 +466,12 @@ int main(int argc, char** argv) {
     int nT = argc > 4 ? atoi(argv[4]) : 35, ng = argc > 5 ? atoi(argv[5]) : 14;
     int nE = argc > 6 ? atoi(argv[6]) : 166, nm = argc > 7 ? atoi(argv[7]) : 67;
     double logg_max = argc > 8 ? atof(argv[8]) : 15.0;
-    return cmd_nsx(argv[2], seed, nT, ng, nE, nm, logg_max);
+    return cmd_nsx(argv[2], seed, nT, ng, nE, nm, logg_max, 0.5, 0.02);
+  }
+  if (argc >= 3 && !strcmp(argv[1], "nsx_bb")) {
+    int nT = argc > 3 ? atoi(argv[3]) : 35, ng = argc > 4 ? atoi(argv[4]) : 14;
+    int nE = argc > 5 ? atoi(argv[5]) : 166, nm = argc > 6 ? atoi(argv[6]) : 67;
+    return cmd_nsx(argv[2], 1, nT, ng, nE, nm, 15.0, 0.0, 0.0);
   }
   if (argc >= 5 && !strcmp(argv[1], "response")) {
     uint64_t seed = argc > 5 ? strtoull(argv[5], 0, 10) : 0;
@@ -469,6 +480,7 @@ int main(int argc, char** argv) {
   fprintf(stderr,
           "usage: ppm_synth lensing OUTDIR [u_min u_max n_u n_cos]\n"
           "       ppm_synth nsx PATH [seed nT ng nE nmu [logg_max]]\n"
+          "       ppm_synth nsx_bb PATH [nT ng nE nmu]\n"
           "       ppm_synth response OUTDIR n_ch n_E [seed]\n");
   return 2;
 }
