@@ -2,7 +2,9 @@
 //
 //   k_ring_setup      one workgroup per (spot, candidate ring): FP64 patch
 //                     membership compacted into runs by wavefront ballot and
-//                     prefix scan; FP64 fine-phase table of flux/I, redshift,
+//                     prefix scan; the ring's two rows of the lensing tables
+//                     staged in LDS for the bilinear look-ups; FP64
+//                     fine-phase table of flux/I, redshift,
 //                     cos(sigma') and time-delayed phase; ring-skip rule, gap
 //                     fill, closure; NSX (logT, log g) bracket.
 //   k_integral_plain  one workgroup per (ring, 128-energy chunk): NSX slice
@@ -128,24 +130,51 @@ __device__ inline int axis_stencil(double x_min, double x_max, int n, double x, 
   return s;
 }
 
-__device__ inline void lens_ca_lf(const DevLensing& L, double u, double cp, double& ca, double& lf) {
-  double au, ac;  // cpu/lensing.hpp:43-60
-  const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, au);
+// The two rows (iu, iu + 1) of the three lensing tables that one ring's
+// bilinear look-ups read (every element of a ring has the same u), staged
+// in LDS by the ring's workgroup: [2][n_cp] cos(alpha), [2][n_cp] lensing
+// factor, [2][n_ca] c dt / R.
+struct LensRows {
+  const double *ca, *lf, *dt;
+  double au;  // fraction of u between rows iu and iu + 1
+};
+
+__device__ inline LensRows stage_lens_rows(const DevLensing& L, double u, double* s, int tid, int nthreads) {
+  LensRows r;
+  const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, r.au);  // cpu/lensing.hpp:43-72
+  double* ca = s;
+  double* lf = s + 2 * L.n_cp;
+  double* dt = s + 4 * L.n_cp;
+  const double* gca = L.ca + (size_t)iu * L.n_cp;
+  const double* glf = L.lf + (size_t)iu * L.n_cp;
+  const double* gdt = L.dt + (size_t)iu * L.n_ca;
+  for (int k = tid; k < 2 * L.n_cp; k += nthreads) {
+    ca[k] = gca[k];
+    lf[k] = glf[k];
+  }
+  for (int k = tid; k < 2 * L.n_ca; k += nthreads) dt[k] = gdt[k];
+  r.ca = ca;
+  r.lf = lf;
+  r.dt = dt;
+  return r;
+}
+
+__device__ inline void lens_ca_lf(const DevLensing& L, const LensRows& R, double cp, double& ca, double& lf) {
+  double ac;  // cpu/lensing.hpp:43-60
   const int ic = uniform_hunt(L.cp_min, L.cp_max, L.n_cp, cp, ac);
-  const double bu = 1. - au, bc = 1. - ac;
-  const double* A = L.ca + (size_t)iu * L.n_cp;
-  const double* F = L.lf + (size_t)iu * L.n_cp;
+  const double au = R.au, bu = 1. - au, bc = 1. - ac;
+  const double* A = R.ca;
+  const double* F = R.lf;
   const int n = L.n_cp;
   ca = bu * bc * A[ic] + au * bc * A[n + ic] + bu * ac * A[ic + 1] + au * ac * A[n + ic + 1];
   lf = bu * bc * F[ic] + au * bc * F[n + ic] + bu * ac * F[ic + 1] + au * ac * F[n + ic + 1];
 }
 
-__device__ inline double lens_cdt(const DevLensing& L, double u, double ca) {
-  double au, ac;  // cpu/lensing.hpp:62-72
-  const int iu = uniform_hunt(L.u_min, L.u_max, L.n_u, u, au);
+__device__ inline double lens_cdt(const DevLensing& L, const LensRows& R, double ca) {
+  double ac;  // cpu/lensing.hpp:62-72
   const int ic = uniform_hunt(L.ca_min, L.ca_max, L.n_ca, ca, ac);
-  const double bu = 1. - au, bc = 1. - ac;
-  const double* D = L.dt + (size_t)iu * L.n_ca;
+  const double au = R.au, bu = 1. - au, bc = 1. - ac;
+  const double* D = R.dt;
   const int n = L.n_ca;
   return bu * bc * D[ic] + au * bc * D[n + ic] + bu * ac * D[ic + 1] + au * ac * D[n + ic + 1];
 }
@@ -310,6 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
   const double R = S.Re * (1. + S.o2 * cz * cz);
   const double dRdth = -2. * S.Re * S.o2 * cz * sz;
   const double u = S.Rs / R, uu = sqrt(1. - u);
+  extern __shared__ __align__(16) unsigned char setup_smem[];
+  const LensRows LR = stage_lens_rows(L, u, reinterpret_cast<double*>(setup_smem), tid, kBlock);
+  __syncthreads();
   const double f = dRdth / (R * uu), ff = sqrt(1. + f * f);
   const double dS = R * R * d_omega * ff;
   const double cos_tau = 1. / ff, sin_tau = f / ff;
@@ -342,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
       }
       const double sin_psi = sqrt(1. - cos_psi * cos_psi);
       double ca, lf;
-      lens_ca_lf(L, u, cos_psi, ca, lf);
+      lens_ca_lf(L, LR, cos_psi, ca, lf);
       const double sa = sqrt(1. - ca * ca);
       const double sa_sp = cos_psi == 1. ? sqrt(lf) : sa / sin_psi;
       {
@@ -359,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
       const double cos_xi = -sa_sp * S.si * sphi;
       const double delta = 1. / (gamma * (1. - beta * cos_xi));
       const double delta3 = delta * delta * delta;
-      const double dt1 = lens_cdt(L, u, ca) * R / kCKmS;
+      const double dt1 = lens_cdt(L, LR, ca) * R / kCKmS;
       double dt2;
       if (ca >= 0) {
         dt2 = dtR;
@@ -855,7 +887,18 @@ int max_dynamic_lds(int device) {
 
 int launch_ring_setup(const DevLensing& L, const DevNsx& N, const PlainLaunch& P, void* stream) {
   if (P.n_tasks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ring_setup, dim3(P.n_tasks), dim3(kBlock), 0, (hipStream_t)stream, L, N, P);
+  // the ring's two rows of the lensing tables, staged in LDS
+  const size_t lds = sizeof(double) * (4 * (size_t)L.n_cp + 2 * (size_t)L.n_ca);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  hipFuncAttributes fa{};
+  if (e == hipSuccess) e = hipFuncGetAttributes(&fa, (const void*)k_ring_setup);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_ring_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            max_dynamic_lds(dev) - (int)fa.sharedSizeBytes);
+  if (e != hipSuccess) return e;
+  if (lds + fa.sharedSizeBytes > (size_t)max_dynamic_lds(dev)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ring_setup, dim3(P.n_tasks), dim3(kBlock), lds, (hipStream_t)stream, L, N, P);
   return hipGetLastError();
 }
 
