@@ -110,6 +110,18 @@ class TuneResult(ctypes.Structure):
                 ("reserved", c_int)]
 
 
+class ResponseSpec(ctypes.Structure):
+    _fields_ = [("rmf", ctypes.POINTER(ctypes.c_double)), ("arf", ctypes.POINTER(ctypes.c_double)),
+                ("ism", ctypes.POINTER(ctypes.c_double)), ("n_e_all", ctypes.c_int), ("n_ch_all", ctypes.c_int),
+                ("n_ism", ctypes.c_int), ("e_lo", ctypes.c_int), ("e_hi", ctypes.c_int), ("ch_lo", ctypes.c_int),
+                ("ch_hi", ctypes.c_int), ("n_eobs", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class LensingSpec(ctypes.Structure):
+    _fields_ = [("u_min", ctypes.c_double), ("u_max", ctypes.c_double), ("n_u", ctypes.c_int),
+                ("n_cos", ctypes.c_int), ("device", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int64), ("running", ctypes.c_int64), ("launches", ctypes.c_int64 * 2),
                 ("overlapped", ctypes.c_int64), ("charged_bytes", ctypes.c_size_t),

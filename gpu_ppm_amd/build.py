@@ -26,7 +26,7 @@ HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wno-unused-result",
               "-I" + os.path.join(ROOT, "include")]
 
 LIB_SOURCES = ["ppm_kernels.hip", "ppm_batch_kernels.hip", "ppm_api.cpp", "ppm_tables.cpp",
-               "ppm_batch.cpp", "ppm_serve.cpp"]
+               "ppm_batch.cpp", "ppm_serve.cpp", "ppm_response_gen.hip", "ppm_lensing_gen.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -81,8 +81,44 @@ def load():
     for f in (L.ppm_set_launch_shape, L.ppm_get_launch_shape):
         f.restype = c_int
         f.argtypes = [c_void_p, POINTER(abi.Grid), c_int, c_int, POINTER(abi.LaunchShape)]
+    L.ppm_lensing_build.restype = c_int
+    L.ppm_lensing_build.argtypes = [POINTER(abi.LensingSpec), POINTER(c_double), POINTER(c_double),
+                                    POINTER(c_double)]
+    L.ppm_response_build.restype = c_int
+    L.ppm_response_build.argtypes = [POINTER(abi.ResponseSpec), POINTER(c_double), POINTER(c_double)]
     _lib = L
     return L
+
+
+def lensing_build(u_min, u_max, n_u, n_cos, device=0):
+    """Lensing tables on the device (ppm_lensing_build): cos(alpha) and the
+    lensing factor on the cos(psi) axis, c dt / R on the cos(alpha) axis,
+    each [n_u][n_cos]."""
+    L = load()
+    sp = abi.LensingSpec(u_min, u_max, n_u, n_cos, device, 0)
+    ca, lf, dt = (np.zeros((n_u, n_cos)) for _ in range(3))
+    st = L.ppm_lensing_build(byref(sp), _dptr(ca), _dptr(lf), _dptr(dt))
+    if st != abi.OK:
+        raise PpmError(st, L.ppm_strerror(st).decode())
+    return ca, lf, dt
+
+
+def response_build(rmf, arf, ism, e_range, ch_range, n_eobs, device=0):
+    """Instrument response on the device (ppm_response_build): rmf
+    [n_e_all][n_ch_all], arf [n_e_all][3] (E_lo, E_hi, area), ism [n_ism][2]
+    (energy, transmission); returns (E_obs [n_eobs], rsp [n_ch][n_eobs])."""
+    L = load()
+    rmf = np.ascontiguousarray(rmf, dtype=np.float64)
+    arf = np.ascontiguousarray(arf, dtype=np.float64)
+    ism = np.ascontiguousarray(ism, dtype=np.float64)
+    sp = abi.ResponseSpec(_dptr(rmf), _dptr(arf), _dptr(ism), rmf.shape[0], rmf.shape[1], ism.shape[0],
+                          e_range[0], e_range[1], ch_range[0], ch_range[1], n_eobs, device)
+    e_obs = np.zeros(n_eobs)
+    rsp = np.zeros((ch_range[1] - ch_range[0], n_eobs))
+    st = L.ppm_response_build(byref(sp), _dptr(e_obs), _dptr(rsp))
+    if st != abi.OK:
+        raise PpmError(st, L.ppm_strerror(st).decode())
+    return e_obs, rsp
 
 
 def _dptr(a):

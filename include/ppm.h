@@ -255,6 +255,46 @@ typedef struct ppm_serve_stats {
 } ppm_serve_stats;
 ppm_status ppm_serve_stats_get(ppm_ctx* ctx, ppm_serve_stats* out);
 
+/* Instrument response built on the device from an instrument's files,
+ * replacing c_ism_inst.py:30-101 (numpy/scipy in the reference): trim to
+ * channels [ch_lo, ch_hi) and energies [e_lo, e_hi), response = area x RMF,
+ * E_obs log-spaced between the first lower and the last upper bin edge (ends
+ * exact), C[j][i] = integral over fine bin j of the not-a-knot cubic spline
+ * through the unit vector e_i on E_obs, ISM transmission (not-a-knot spline
+ * at the bin centres) to the 5th power, rsp = (response x ISM^5) @ C with
+ * |x| < 1e-300 set to 0.  Host arrays in, host arrays out; the result is
+ * what tables/ism_inst/<setting>/{num,E_obs,rsp}.txt hold. */
+typedef struct ppm_response_spec {
+  const double* rmf;  /* [n_e_all][n_ch_all]: the RMF file's last column, energy-major */
+  const double* arf;  /* [n_e_all][3]: E_lo, E_hi, area                              */
+  const double* ism;  /* [n_ism][2]: energy, transmission (ISM file columns 0 and 2)  */
+  int n_e_all, n_ch_all, n_ism;
+  int e_lo, e_hi, ch_lo, ch_hi;  /* trim ranges (c_ism_inst.py:9-12: 0,1800 and 30,300) */
+  int n_eobs;                    /* observed energies (std 128, high 256, ultra 512)    */
+  int device;                    /* HIP device ordinal                                 */
+} ppm_response_spec;
+ppm_status ppm_response_build(const ppm_response_spec* spec, double* e_obs /* [n_eobs] */,
+                              double* rsp /* [ch_hi - ch_lo][n_eobs] */);
+
+/* Lensing tables built on the device, replacing c_lensing.py:47-155 (scipy
+ * quad, numpy.gradient and PchipInterpolator in the reference): for u on
+ * [u_min, u_max] (n_u nodes) the alpha march of step pi / (8 n_cos) until psi
+ * passes acos(-0.9), psi(alpha) and c dt / R by adaptive Gauss-Kronrod 7/15
+ * quadrature (relative tolerance 1e-12), the lensing factor
+ * d cos(alpha) / d cos(psi) by the non-uniform second-order difference, and
+ * cos(alpha), the lensing factor (on cos psi in [-0.9, 1]) and c dt / R (on
+ * cos alpha in [-0.2, 1]) on n_cos uniform nodes: the three matrices of
+ * tables/lensing/<setting>/ (std: 256 u in 0.3-0.5, 512 nodes).  Outputs are
+ * host arrays [n_u][n_cos]; u_max < 0.568 (single images). */
+typedef struct ppm_lensing_spec {
+  double u_min, u_max;
+  int n_u, n_cos;
+  int device; /* HIP device ordinal */
+  int reserved;
+} ppm_lensing_spec;
+ppm_status ppm_lensing_build(const ppm_lensing_spec* spec, double* cos_alpha, double* lensing_factor,
+                             double* cdt_over_R);
+
 /* Fit of one star's folded counts m[c][p] to data d[c][p]. */
 typedef struct ppm_fit {
   double chi2;        /* sum (d - m)^2 / m with model and data scaled so that the model sums
