@@ -318,6 +318,11 @@ __device__ inline float lane_next(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
+// Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 receives 0.
+__device__ inline float lane_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
 // floor(t) as an integer (one instruction).
 __device__ inline int floor_to_int(float t) {
   int r;
@@ -442,7 +447,11 @@ __device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& 
 // per-wavefront LDS table instead of cross-lane reads: per row set each lane
 // writes its row's four coefficients (one ds_write_b128) and each energy
 // reads its row's (one ds_read_b128), 2 + EPL LDS instructions per patch in
-// place of 4 EPL ds_bpermute.  Both sets' coefficients and every energy's
+// place of 4 EPL ds_bpermute.  Here lane l forms the coefficients of the
+// row it contracted itself (row i0 - 1 + l: a0 is its own value, the first
+// difference looks back one lane, the higher ones forward), so entry l of
+// the table belongs to row i0 - 1 + l and an energy of row f reads entry
+// f - i0 + 1; lane 0 (no lane behind it) is never read, since f >= i0.  Both sets' coefficients and every energy's
 // row and x are formed first, then the sets go through the 64-entry table in
 // turn with all EPL reads in flight before the first result is used: a
 // wavefront's LDS instructions complete in issue order, so the reads of one
@@ -462,22 +471,21 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
 #pragma unroll
   for (int s = 0; s < SETS; ++s) {
     const float U = w[0] * p[s][0] + w[1] * p[s][1] + w[2] * p[s][2] + w[3] * p[s][3];
-    const float U1 = lane_next(U);
     if (kLin) {
-      a[s] = make_float4(U1, lane_next(U1) - U1, 0.f, 0.f);
+      a[s] = make_float4(U, lane_next(U) - U, 0.f, 0.f);
     } else {
-      const float d1 = U1 - U;
+      const float d1 = U - lane_prev(U);
       const float D2 = lane_next(d1) - d1;
       const float D3 = lane_next(D2) - D2;
       const float a2 = 0.5f * D2, a3 = (1.f / 6.f) * D3;
-      a[s] = make_float4(U1, d1 + a2 - a3, a2, a3);
+      a[s] = make_float4(U, d1 + a2 - a3, a2, a3);
     }
   }
   int src[EPL];
   float x[EPL];
   // the clamps of the row f in the float domain (floor and med3 of a float,
-  // f exact), the set's first row folded into the table pointer (s_T - i0
-  // stays inside the dynamic LDS block: the table lies behind the ring
+  // f exact), the set's first row folded into the table pointer (s_T + 1 -
+  // i0 stays inside the dynamic LDS block: the table lies behind the ring
   // tables and the slice, far more than 16 i0 <= 16 EAxis::i0_max bytes)
   const float f_lo = (float)ax.f_lo, f_hi = (float)ax.f_hi;
   const float4* T[SETS];
@@ -486,7 +494,7 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
     const bool clamp = (CM >> s) & 1;
     const int i0 = clamp ? readlane_i(pr.i0[s], q) : 0;
     const float txs = clamp ? 0.f : readlane_f(pr.txs[s], q);
-    T[s] = s_T - i0;
+    T[s] = s_T + 1 - i0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int u = s * G + g;
