@@ -251,11 +251,11 @@ __device__ inline bool nsx_eval_wave(const float (&le)[EPL], float tx, int im, c
 // Wave-uniform description of a wavefront's energies on the local axis.  A
 // lane's EPL energies (u = 0..EPL-1, energy u*64 + lane) are split into SETS
 // row sets of EPL/SETS consecutive u; each set spans its own window of slice
-// rows, contracted separately by nsx_eval_rows.
+// rows, contracted separately by nsx_eval_table.
 template <int SETS>
 struct WaveEnergies {
   float le_first[SETS], le_last[SETS];  // lane 0's first and lane 63's last energy of each set
-  bool fast;  // ascending and every set spanning at most 58 rows: nsx_eval_rows applies
+  bool fast;  // ascending and every set spanning at most 58 rows: nsx_eval_table applies
 };
 
 template <int EPL, int SETS>
@@ -274,7 +274,7 @@ __device__ inline WaveEnergies<SETS> wave_energies(const float (&le)[EPL], int l
 }
 
 // Classification of one patch, made once in its set-up lane.  When the
-// wavefront's energies allow nsx_eval_rows (WaveEnergies::fast) the patch is
+// wavefront's energies allow nsx_eval_table (WaveEnergies::fast) the patch is
 // evaluated there, each row set without clamps when none of its energies
 // reaches a clamp of the axis or of the row f (checked on the set's first and
 // last energy, the evaluation being monotone in the energy), with them
@@ -330,25 +330,6 @@ __device__ inline int floor_to_int(float t) {
   return r;
 }
 
-// Contracted evaluation of NP patches at once (independent chains, for
-// latency hiding), every patch of a wavefront with WaveEnergies::fast and,
-// with CLAMP = false, no bit of PatchRows::clamp; patch j is the one set up in lane
-// q[j] (pv, pr).  For energy t = le + tx (clamped to the axis ends) the row
-// is f = floor(t) (clamped to EAxis::f_lo..f_hi) and x = t - f, in [0, 1)
-// away from the clamps.  For each row set, lane l contracts slice row
-// i0 - 1 + l over the patch's mu stencil, takes the next rows from its
-// neighbours and forms the power-basis coefficients of the polynomial in x of
-// row f = i0 + l: the cubic through rows f-1..f+2 at the nodes x = -1, 0, 1, 2,
-//   I(x) = a0 + x (a1 + x (a2 + x a3))
-// (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256), or the
-// line through rows f, f+1, I(x) = a0 + x a1 (cpu/nsx.hpp:304-316); each
-// energy then fetches the coefficients of its row from the lane that holds
-// them.  The differences of lane l + 1 are the next differences of lane l,
-// so each order takes one shifted subtraction.  Away from the clamps an
-// energy's lane and x come from t - i0 = le + txs directly.  EM = kECubicClip
-// sets negative intensities to 0 (use_sg, cpu/nsx.hpp:201; the weights carry
-// the positive flux/I x weight, so the sign is the interpolant's).
-//
 // nsx_load_rows issues the slice-row reads of patch q (lane l: row i0 - 1 + l
 // of each set at the patch's four mu columns); a caller may issue them for
 // the next patch before evaluating the current one.
@@ -363,100 +344,33 @@ __device__ inline void nsx_load_rows(const PatchRows<SETS>& pr, int q, const flo
   }
 }
 
-template <int EPL, int NP, bool CLAMP, int SETS, int EM>
-__device__ inline void nsx_eval_loaded(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
-                                       const int (&q)[NP], const EAxis& ax, const float (&p)[NP][SETS][4], int lane,
-                                       float (&out)[NP][EPL]) {
-  constexpr int G = EPL / SETS;
-  constexpr bool kLin = EM == kELinear;
-  float a0[NP][SETS], a1[NP][SETS], a2[NP][SETS], a3[NP][SETS];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    float w[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[j]);
-#pragma unroll
-    for (int s = 0; s < SETS; ++s) {
-      const float U = w[0] * p[j][s][0] + w[1] * p[j][s][1] + w[2] * p[j][s][2] + w[3] * p[j][s][3];
-      const float U1 = lane_next(U);
-      if (kLin) {
-        a0[j][s] = U1;                   // row f
-        a1[j][s] = lane_next(U1) - U1;   // row f+1 - row f
-        a2[j][s] = a3[j][s] = 0.f;
-      } else {
-        const float d1 = U1 - U;                // U(0) - U(-1)
-        const float D2 = lane_next(d1) - d1;    // second difference at x = 0
-        const float D3 = lane_next(D2) - D2;    // third difference
-        a2[j][s] = 0.5f * D2;
-        a3[j][s] = (1.f / 6.f) * D3;
-        a1[j][s] = d1 + a2[j][s] - a3[j][s];
-        a0[j][s] = U1;
-      }
-    }
-  }
-  float x[NP][EPL], c[NP][EPL][4];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    float tx = 0.f, txs[SETS];
-    int i0[SETS];
-#pragma unroll
-    for (int s = 0; s < SETS; ++s) {
-      if (CLAMP) {
-        i0[s] = readlane_i(pr.i0[s], q[j]);
-      } else {
-        txs[s] = readlane_f(pr.txs[s], q[j]);
-      }
-    }
-    if (CLAMP) tx = readlane_f(pv.tx, q[j]);
-#pragma unroll
-    for (int u = 0; u < EPL; ++u) {
-      const int s = u / G;
-      int src;
-      if (CLAMP) {
-        const float t = __builtin_amdgcn_fmed3f(le[u] + tx, ax.t_lo, ax.t_hi);
-        const int f = med3i(floor_to_int(t), ax.f_lo, ax.f_hi);
-        x[j][u] = t - (float)f;
-        src = (f - i0[s]) << 2;
-      } else {
-        const float t = le[u] + txs[s];
-        src = floor_to_int(t) << 2;
-        x[j][u] = __builtin_amdgcn_fractf(t);
-      }
-      c[j][u][0] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a0[j][s])));
-      c[j][u][1] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a1[j][s])));
-      if (!kLin) {
-        c[j][u][2] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a2[j][s])));
-        c[j][u][3] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(a3[j][s])));
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < NP; ++j)
-#pragma unroll
-    for (int u = 0; u < EPL; ++u) {
-      float r;
-      if (kLin)
-        r = fmaf(x[j][u], c[j][u][1], c[j][u][0]);
-      else
-        r = fmaf(x[j][u], fmaf(x[j][u], fmaf(x[j][u], c[j][u][3], c[j][u][2]), c[j][u][1]), c[j][u][0]);
-      out[j][u] = EM == kECubicClip ? fmaxf(r, 0.f) : r;
-    }
-}
-
-// The same evaluation for one patch with the coefficients passed through a
-// per-wavefront LDS table instead of cross-lane reads: per row set each lane
-// writes its row's four coefficients (one ds_write_b128) and each energy
-// reads its row's (one ds_read_b128), 2 + EPL LDS instructions per patch in
-// place of 4 EPL ds_bpermute.  Here lane l forms the coefficients of the
-// row it contracted itself (row i0 - 1 + l: a0 is its own value, the first
-// difference looks back one lane, the higher ones forward), so entry l of
-// the table belongs to row i0 - 1 + l and an energy of row f reads entry
-// f - i0 + 1; lane 0 (no lane behind it) is never read, since f >= i0.  Both sets' coefficients and every energy's
-// row and x are formed first, then the sets go through the 64-entry table in
-// turn with all EPL reads in flight before the first result is used: a
-// wavefront's LDS instructions complete in issue order, so the reads of one
-// set precede the next set's writes.  Bit s of CM selects the clamped
-// evaluation for set s (PatchRows::clamp).
+// Contracted evaluation of one patch for every energy of a wavefront with
+// WaveEnergies::fast, the patch set up in lane q (pv, pr) and its slice rows
+// read by nsx_load_rows into p.  For energy t = le + tx (clamped to the axis
+// ends) the row is f = floor(t) (clamped to EAxis::f_lo..f_hi) and x = t - f,
+// in [0, 1) away from the clamps.  For each row set, lane l contracts slice
+// row i0 - 1 + l over the patch's mu stencil and forms the power-basis
+// coefficients of the polynomial in x of that row f: the cubic through rows
+// f-1..f+2 at the nodes x = -1, 0, 1, 2,
+//   I(x) = a0 + x (a1 + x (a2 + x a3))
+// (the polynomial and the clamps of cpu/nsx.hpp:159-167, 252-256), or the
+// line through rows f, f+1, I(x) = a0 + x a1 (cpu/nsx.hpp:304-316).  a0 is
+// the lane's own value, the first difference looks back one lane and the
+// higher ones forward: the differences of lane l + 1 are the next
+// differences of lane l, so each order takes one shifted subtraction.
+// Each lane writes its row's four coefficients to a per-wavefront LDS table
+// (one ds_write_b128) and each energy reads its row's (one ds_read_b128):
+// entry l belongs to row i0 - 1 + l, an energy of row f reads entry
+// f - i0 + 1, and lane 0 (no lane behind it) is never read since f >= i0.
+// Away from the clamps an energy's entry and x come from t - i0 = le + txs
+// directly.  Both sets' coefficients and every energy's row and x are
+// formed first, then the sets go through the 64-entry table in turn with
+// all EPL reads in flight before the first result is used: a wavefront's
+// LDS instructions complete in issue order, so the reads of one set precede
+// the next set's writes.  Bit s of CM selects the clamped evaluation for
+// set s (PatchRows::clamp).  EM = kECubicClip sets negative intensities to
+// 0 (use_sg, cpu/nsx.hpp:201; the weights carry the positive flux/I x
+// weight, so the sign is the interpolant's).
 template <int EPL, int CM, int SETS, int EM>
 __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr, int q,
                                       const EAxis& ax, const float (&p)[SETS][4], int lane, float4* s_T,
@@ -527,18 +441,7 @@ __device__ inline void nsx_eval_table(const float (&le)[EPL], const PatchVals& p
   }
 }
 
-template <int EPL, int NP, bool CLAMP, int SETS, int EM>
-__device__ inline void nsx_eval_rows(const float (&le)[EPL], const PatchVals& pv, const PatchRows<SETS>& pr,
-                                     const int (&q)[NP], const EAxis& ax, const float* s_S, int lane,
-                                     float (&out)[NP][EPL]) {
-  // all LDS reads of a stage are issued before their results are used
-  float p[NP][SETS][4];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) nsx_load_rows(pr, q[j], s_S, lane, p[j]);
-  nsx_eval_loaded<EPL, NP, CLAMP, SETS, EM>(le, pv, pr, q, ax, p, lane, out);
-}
-
-// One patch whose wavefront energies span too many rows for nsx_eval_rows:
+// One patch whose wavefront energies span too many rows for nsx_eval_table:
 // contracted evaluation with clamps (nsx_eval_wave) when they fit the
 // scratch, else per energy (nsx_eval).
 template <int EPL, int EM>

@@ -533,12 +533,14 @@ __host__ __device__ inline LdsLayout integral_layout(int nf1, int nsx_rows) {
 
 constexpr int kEPL = 2;                     // energies per lane
 constexpr int kEPerWave = 64 * kEPL;        // energies per wavefront / workgroup
-constexpr int kNP = 4;                      // patches evaluated together
+constexpr int kNP = 4;                      // patches per round of the per-energy path
 
 // One workgroup per (ring, 128-energy chunk); its 4 wavefronts take the
 // phase bins round-robin.  Within a wavefront each lane sets up one patch in
-// FP64 and keeps it in registers; the evaluation loop broadcasts a patch
-// with readlane and every lane evaluates it at its two energies.
+// FP64 and keeps it in registers; the evaluation loop broadcasts a patch's
+// values with readlane, contracts its mu stencil over one slice row per lane
+// and evaluates every lane's two energies from the wavefront's table of cubic
+// coefficients (nsx_eval_table), as the batched kernel does.
 template <int ATM, int EM>
 __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx N, PlainLaunch P) {
   const int task = blockIdx.x, chunk = blockIdx.y;
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
           patch_setup<ATM>(pp, pt.w, R, s_mu, N.mu_den, N.nm, N.mu_min, mu_mode, txm, pv);
         }
         // ---- evaluation: every lane evaluates each visible patch at its
-        // energies, kNP patches at a time
+        // energies
         float bsum[kEPL];
 #pragma unroll
         for (int u = 0; u < kEPL; ++u) bsum[u] = 0.f;
@@ -659,45 +661,51 @@ __global__ __launch_bounds__(kBlock) void k_integral_plain(DevLensing L, DevNsx 
         PatchRows<1> pr{};
         if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
         const uint64_t interior = __ballot(pv.im >= 0 && pr.clamp == 0);
+        if (ATM == PPM_ATM_NSX && we.fast) {
+          // one patch at a time through the wavefront's coefficient table;
+          // the slice rows of the next patch are read while the current one
+          // is evaluated (two row buffers, used in turn)
+          auto step = [&](const float (&p_cur)[1][4], float (&p_next)[1][4]) {
+            const int q = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
+            float val[kEPL];
+            if (__builtin_expect((interior >> q) & 1, 1))
+              nsx_eval_table<kEPL, 0, 1, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+            else
+              nsx_eval_table<kEPL, 1, 1, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+#pragma unroll
+            for (int u = 0; u < kEPL; ++u) bsum[u] += val[u];
+          };
+          float p_a[1][4], p_b[1][4];
+          if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a);
+          while (todo) {
+            step(p_a, p_b);
+            if (!todo) break;
+            step(p_b, p_a);
+          }
+        }
+        // energies whose rows exceed the table (NSX), or the blackbody:
+        // kNP patches per round
         while (todo) {
-          int qs[kNP];  // a missing patch repeats the first
-          bool all_interior = true;
+          int qs[kNP];  // -1: no patch
 #pragma unroll
           for (int j = 0; j < kNP; ++j) {
             qs[j] = todo ? __builtin_ctzll(todo) : -1;
             todo &= todo - 1;
-            if (qs[j] >= 0) all_interior = all_interior && ((interior >> qs[j]) & 1);
           }
-          if (ATM == PPM_ATM_NSX) {
-            int q[kNP];
 #pragma unroll
-            for (int j = 0; j < kNP; ++j) q[j] = qs[j] >= 0 ? qs[j] : qs[0];
-            float val[kNP][kEPL];
-            if (we.fast) {
-              if (all_interior)
-                nsx_eval_rows<kEPL, kNP, false, 1, EM>(le, pv, pr, q, ax, s_S, lane, val);
-              else
-                nsx_eval_rows<kEPL, kNP, true, 1, EM>(le, pv, pr, q, ax, s_S, lane, val);
+          for (int j = 0; j < kNP; ++j) {
+            if (qs[j] < 0) break;
+            if (ATM == PPM_ATM_NSX) {
+              float w[4], val[kEPL];
+#pragma unroll
+              for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], qs[j]);
+              nsx_eval_edge<kEPL, EM>(le, asc, readlane_f(pv.tx, qs[j]), readlane_i(pv.im, qs[j]), w, ax, s_S, s_C,
+                                      lane, val);
+#pragma unroll
+              for (int u = 0; u < kEPL; ++u) bsum[u] += val[u];
             } else {
-#pragma unroll
-              for (int j = 0; j < kNP; ++j) {
-                if (qs[j] < 0) break;
-                float w[4];
-#pragma unroll
-                for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q[j]);
-                nsx_eval_edge<kEPL, EM>(le, asc, readlane_f(pv.tx, q[j]), readlane_i(pv.im, q[j]), w, ax, s_S, s_C,
-                                        lane, val[j]);
-              }
-            }
-#pragma unroll
-            for (int j = 0; j < kNP; ++j)
-              if (qs[j] >= 0)
-#pragma unroll
-                for (int u = 0; u < kEPL; ++u) bsum[u] += val[j][u];
-          } else {
-#pragma unroll
-            for (int j = 0; j < kNP; ++j) {
-              if (qs[j] < 0) break;
               const float c = readlane_f(pv.w[0], qs[j]), z = readlane_f(pv.zf, qs[j]);
 #pragma unroll
               for (int u = 0; u < kEPL; ++u) {
