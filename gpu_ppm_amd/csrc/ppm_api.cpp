@@ -496,6 +496,26 @@ ppm_status eval_plain_rows(ppm_ctx* c, Lane& L, const ppm_star& st, const ppm_gr
   return PPM_OK;
 }
 
+// PPM_METHOD_AUTO for one star: the batched path (phase-shift dedupe, one
+// star per launch) where it is the faster one on MI355X and can take the
+// grid, else the plain path.  Measured per profile (64 bins x 2 GL, 360
+// energies): N_side 128 plain 1.5 ms / batched 2.1 ms, 256 4.0 / 3.6, 1024
+// 40 / 11, 2048 152 / 27; with 32 bins and 128 energies the batched path is
+// ahead from N_side 128 (1.2 / 1.0 ms).
+static int auto_single_method(const ppm_ctx* c, const ppm_star& star, const ppm_grid& g,
+                              const std::vector<double>& energies) {
+  const int n_side_min = energies.size() <= 128 ? 128 : 256;
+  if (g.n_side < n_side_min) return PPM_METHOD_PLAIN;
+  for (size_t q = 1; q < energies.size(); ++q)
+    if (!(energies[q] > energies[q - 1])) return PPM_METHOD_PLAIN;
+  BatchGroup G;
+  G.stars = &star;
+  G.n = 1;
+  G.g = g;
+  G.energies = energies;
+  return batch_groups_fit(c, {G}) ? PPM_METHOD_BATCHED : PPM_METHOD_PLAIN;
+}
+
 ppm_status eval_any(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm_grid* grid, double* d_flux,
                     double* d_counts, ppm_stats* stats) {
   if (!c || !stars || !grid || n < 1 || !d_flux) return fail(c, PPM_ERR_ARG, "null argument");
@@ -509,7 +529,7 @@ ppm_status eval_any(ppm_ctx* c, Lane& L, const ppm_star* stars, int n, const ppm
   phase_points(*grid, pts);
   if (stats) std::memset(stats, 0, sizeof *stats);
   int method = grid->method;
-  if (method == PPM_METHOD_AUTO) method = n == 1 ? PPM_METHOD_PLAIN : PPM_METHOD_BATCHED;
+  if (method == PPM_METHOD_AUTO) method = n == 1 ? auto_single_method(c, stars[0], *grid, energies) : PPM_METHOD_BATCHED;
   if (method == PPM_METHOD_BATCHED)
     return eval_batched(c, L, stars, n, *grid, energies, pts, d_flux, d_counts, stats);
   if (method != PPM_METHOD_PLAIN) return fail(c, PPM_ERR_ARG, "unknown method");

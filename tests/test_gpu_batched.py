@@ -142,3 +142,19 @@ def test_energy_chunks_after_a_larger_batch(ctx, orc):
         e = rel_err(flux[k], ref)
         print("  star %d: %.2e (high chunk %.2e)" % (k, e, rel_err(flux[k][:, 384:], ref[:, 384:])), flush=True)
         assert e <= TOL, (k, e)
+
+
+def test_auto_method_for_one_star(ctx):
+    """PPM_METHOD_AUTO with one star takes the batched path from N_side 256
+    (ascending energies), the plain path below it and for unordered energies:
+    the results equal those of the method named explicitly."""
+    (star,) = S.sample_priors(1, seed=4)
+    cases = [(256, dict(response=0), abi.METHOD_BATCHED), (64, dict(response=0), abi.METHOD_PLAIN),
+             (256, dict(response=-1, energies=[2.0, 1.0, 3.0]), abi.METHOD_PLAIN)]
+    for n_side, kw, expect in cases:
+        print("case: AUTO, one star, N_side %d, %s" % (n_side, kw), flush=True)
+        fa, _, sa = ctx.eval([star], batch_grid(n_side, 16, method=abi.METHOD_AUTO, **kw))
+        fe, _, se = ctx.eval([star], batch_grid(n_side, 16, method=expect, **kw))
+        assert np.array_equal(fa, fe)
+        assert sa.n_eval_done == se.n_eval_done
+        assert (sa.n_eval_done < sa.n_eval_ref) == (expect == abi.METHOD_BATCHED)
