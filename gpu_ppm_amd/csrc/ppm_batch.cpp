@@ -304,8 +304,11 @@ static int launch_epl(const std::vector<BatchGroup>& groups) {
   return ne <= 128 ? 2 : 6;
 }
 
+// LDS layout of a launch over `groups`: the largest fine table, slice rows
+// and coverage bitmap among them.  Returns the LDS bytes per workgroup with
+// the fine tables staged in LDS (fine_lds) or read from global memory.
 static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& groups, int atm, int rows_override,
-                              int& nsx_rows, int& nf_max, int& cov_words) {
+                              bool fine_lds, int& nsx_rows, int& nf_max, int& cov_words) {
   const int per_item = batch_energies(launch_epl(groups));
   nsx_rows = nf_max = cov_words = 0;
   const double h = (c->N.logE_max - c->N.logE_min) / std::max(c->N.nE - 1, 1);
@@ -326,13 +329,13 @@ static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& g
     if (rows_override) nsx_rows = rows_override;
     nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
   }
-  return batch_lds_bytes(nf_max, atm, nsx_rows, cov_words, kEvCap);
+  return batch_lds_bytes(nf_max, atm, nsx_rows, cov_words, kEvCap, fine_lds);
 }
 
 bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups) {
   if (groups.empty() || !groups[0].n) return false;
   int r, f, w;
-  return launch_geometry(c, groups, groups[0].stars[0].atmosphere, 0, r, f, w) <= (size_t)c->max_lds;
+  return launch_geometry(c, groups, groups[0].stars[0].atmosphere, 0, false, r, f, w) <= (size_t)c->max_lds;
 }
 
 static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats,
@@ -428,12 +431,21 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   const int n_tasks = (int)tasks.size();
 
   // ---- launch geometry: the LDS layout of the largest grid
+  // the fine tables go to LDS unless that lowers the occupancy
   int nsx_rows, nf_max, cov_words;
-  const size_t lds = launch_geometry(c, groups, atm, rows_override, nsx_rows, nf_max, cov_words);
-  if (lds > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
-  if ((e = batched_prepare(atm, epl, emode, c->max_lds))) return hip_fail(c, e, "kernel attributes");
-  int wg_per_cu = 0;
-  if (batched_occupancy(atm, epl, emode, lds, &wg_per_cu) || wg_per_cu < 1) wg_per_cu = 1;
+  const size_t lds_global = launch_geometry(c, groups, atm, rows_override, false, nsx_rows, nf_max, cov_words);
+  const size_t lds_staged = launch_geometry(c, groups, atm, rows_override, true, nsx_rows, nf_max, cov_words);
+  if (lds_global > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
+  int occ_global = 0, occ_staged = 0;
+  if ((e = batched_prepare(atm, epl, emode, false, c->max_lds))) return hip_fail(c, e, "kernel attributes");
+  if (batched_occupancy(atm, epl, emode, false, lds_global, &occ_global) || occ_global < 1) occ_global = 1;
+  if (lds_staged <= (size_t)c->max_lds) {
+    if ((e = batched_prepare(atm, epl, emode, true, c->max_lds))) return hip_fail(c, e, "kernel attributes");
+    if (batched_occupancy(atm, epl, emode, true, lds_staged, &occ_staged)) occ_staged = 0;
+  }
+  const bool fine_lds = occ_staged >= occ_global;
+  const size_t lds = fine_lds ? lds_staged : lds_global;
+  int wg_per_cu = fine_lds ? occ_staged : occ_global;
   L.occ_wg_max = wg_per_cu;
   ppm_launch_shape shape{0, 0};
   if (L.forced) {
@@ -619,6 +631,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   B.epl = epl;
   B.atmosphere = atm;
   B.emode = emode;
+  B.fine_lds = fine_lds;
   B.persistent = !L.per_item;
   const int n_wg = L.per_item ? std::max(1, n_items) : std::max(1, std::min(slots, n_items));
 

@@ -67,19 +67,20 @@ struct BatchLaunch {
   int persistent;          // 1: resident workgroups take items from the atomic queue;
                            // 0: one workgroup per item (workgroup b takes queue position b)
   int emode;               // NsxEnergyMode shared by the launch's stars
+  int fine_lds;            // 1: the ring's fine tables are staged in LDS, 0: read from global memory
   unsigned long long* rows_flag;  // bit 1 set when a ring needs more NSX rows than staged
 };
 
-size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap);
+size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap, bool fine_lds);
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 int batch_energies(int epl);  // energies per item
-int batched_prepare(int atmosphere, int epl, int emode, int lds_max);  // outside any stream capture
+int batched_prepare(int atmosphere, int epl, int emode, bool fine_lds, int lds_max);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);
 int launch_zero_u64(unsigned long long* p, int n, void* stream);  // a kernel, not a memset (graph capture)
 // flux[s - s0][b][e] for the stars [s0, s1) of one grid (G, its host copy).
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream);
-int batched_occupancy(int atmosphere, int epl, int emode, size_t lds, int* wg_per_cu);
+int batched_occupancy(int atmosphere, int epl, int emode, bool fine_lds, size_t lds, int* wg_per_cu);
 
 }  // namespace ppm

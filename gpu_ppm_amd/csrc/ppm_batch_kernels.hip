@@ -132,7 +132,10 @@ __global__ void k_queue_reset(int* q, unsigned long long* n_evals) {
   }
 }
 
-template <int ATM, int EPL, int EM>
+// FL: the ring's fine-phase tables are staged in LDS (FL) or read from
+// global memory where staging them would cost a resident workgroup per CU
+// (large n_fine); the host picks the instance with the higher occupancy.
+template <int ATM, int EPL, int EM, bool FL>
 __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, BatchLaunch B) {
   constexpr int kBEPL = EPL;
   constexpr int kBEnergies = 64 * EPL;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   const int wp = wave;  // position part
   extern __shared__ __align__(16) unsigned char smem[];
   const int rows_alloc = ATM == PPM_ATM_NSX ? B.nsx_rows : 0;
-  const BLds lay = batch_layout(B.n_fine_max + 1, rows_alloc, B.cov_words, B.ev_cap);
+  const BLds lay = batch_layout(FL ? B.n_fine_max + 1 : 0, rows_alloc, B.cov_words, B.ev_cap);
   double* s_ph = reinterpret_cast<double*>(smem + lay.ph);
   double* s_mu = reinterpret_cast<double*>(smem + lay.mu);
   float* s_fI = reinterpret_cast<float*>(smem + lay.fI);
@@ -202,16 +205,15 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const RingTask T = B.tasks[task];
       const SpotParams& S = B.spots[T.spot];
       __syncthreads();  // previous ring's LDS is no longer read
-      {
-        const double* gph = G.ph + (size_t)(task - G.task0) * nf1;
-        const float* gf = G.fine + (size_t)(task - G.task0) * 3 * nf1;
+      const double* gph = G.ph + (size_t)(task - G.task0) * nf1;
+      const float* gf = G.fine + (size_t)(task - G.task0) * 3 * nf1;
+      if (FL)
         for (int k = tid; k < nf1; k += kBThreads) {
           s_ph[k] = gph[k];
           s_fI[k] = gf[k];
           s_zf[k] = gf[nf1 + k];
           s_cs[k] = gf[2 * nf1 + k];
         }
-      }
       int r_lo = 0, n_rows = 0;
       EAxis ax{0.f, 0.f, 0, 0, 1};
       if (ATM == PPM_ATM_NSX) {
@@ -236,7 +238,8 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                       (float)(inv_h * 0.30102999566398120)};
       const float inv_kT = (float)S.inv_kT;
       const int mu_mode = nsx_mu_mode(S.nsx_interp);
-      const RingTab R{s_ph, s_fI, s_zf, s_cs, nf1};
+      const RingTab R = FL ? RingTab{s_ph, s_fI, s_zf, s_cs, nf1} : RingTab{gph, gf, gf + nf1, gf + 2 * nf1, nf1};
+      const double ph0 = R.ph[0];
       const bool point = T.ring == 0;
       const bool ring_hidden = Dp->hidden != 0;
       const int n = point ? 1 : Dp->n_pix;
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         auto phase_of = [&](int k) {
           const double v = (k + c_o) * inv_n + 1.0;
           double pp = v - floor(v);
-          if (pp < s_ph[0]) pp += 1.;
+          if (pp < ph0) pp += 1.;
           return pp;
         };
         const int per_group = max(1, B.ev_cap / (4 * n_runs));
@@ -558,8 +561,8 @@ __global__ void k_reduce_batched(BatchLaunch B, int s0, int s1, int n_phase, int
 
 }  // namespace
 
-size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap) {
-  return batch_layout(n_fine + 1, atmosphere == PPM_ATM_NSX ? nsx_rows : 0, cov_words, ev_cap).total;
+size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap, bool fine_lds) {
+  return batch_layout(fine_lds ? n_fine + 1 : 0, atmosphere == PPM_ATM_NSX ? nsx_rows : 0, cov_words, ev_cap).total;
 }
 
 int batch_threads() { return kBThreads; }
@@ -568,26 +571,30 @@ size_t batch_seg_doubles() { return (size_t)kBWP * kMaxEPL * 64; }
 int batch_energies(int epl) { return 64 * epl; }
 
 namespace {
-template <int ATM, int EM>
+template <int ATM, int EM, bool FL>
 const void* batched_fn_epl(int epl) {
-  return epl == 2 ? (const void*)k_integral_batched<ATM, 2, EM> : (const void*)k_integral_batched<ATM, 6, EM>;
+  return epl == 2 ? (const void*)k_integral_batched<ATM, 2, EM, FL> : (const void*)k_integral_batched<ATM, 6, EM, FL>;
+}
+template <int ATM, int EM>
+const void* batched_fn_fl(int epl, bool fine_lds) {
+  return fine_lds ? batched_fn_epl<ATM, EM, true>(epl) : batched_fn_epl<ATM, EM, false>(epl);
 }
 // The kernel instance of a launch: atmosphere, energies per lane and the
 // energy-axis mode of the NSX interpolant.
-const void* batched_fn(int atmosphere, int epl, int emode) {
-  if (atmosphere != PPM_ATM_NSX) return batched_fn_epl<PPM_ATM_BLACKBODY, kECubic>(epl);
+const void* batched_fn(int atmosphere, int epl, int emode, bool fine_lds) {
+  if (atmosphere != PPM_ATM_NSX) return batched_fn_fl<PPM_ATM_BLACKBODY, kECubic>(epl, fine_lds);
   switch (emode) {
-    case kECubicClip: return batched_fn_epl<PPM_ATM_NSX, kECubicClip>(epl);
-    case kELinear: return batched_fn_epl<PPM_ATM_NSX, kELinear>(epl);
-    default: return batched_fn_epl<PPM_ATM_NSX, kECubic>(epl);
+    case kECubicClip: return batched_fn_fl<PPM_ATM_NSX, kECubicClip>(epl, fine_lds);
+    case kELinear: return batched_fn_fl<PPM_ATM_NSX, kELinear>(epl, fine_lds);
+    default: return batched_fn_fl<PPM_ATM_NSX, kECubic>(epl, fine_lds);
   }
 }
 }  // namespace
 
 // The dynamic LDS limit is set to the device's maximum (lds_max), the same
 // value from every lane, so concurrent launches never lower it for another.
-int batched_prepare(int atmosphere, int epl, int emode, int lds_max) {
-  const void* fn = batched_fn(atmosphere, epl, emode);
+int batched_prepare(int atmosphere, int epl, int emode, bool fine_lds, int lds_max) {
+  const void* fn = batched_fn(atmosphere, epl, emode, fine_lds);
   hipFuncAttributes fa{};
   const hipError_t e = hipFuncGetAttributes(&fa, fn);
   if (e != hipSuccess) return e;
@@ -597,12 +604,12 @@ int batched_prepare(int atmosphere, int epl, int emode, int lds_max) {
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, B.queue, B.n_evals);
-  const size_t lds = batch_lds_bytes(B.n_fine_max, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap);
+  const size_t lds = batch_lds_bytes(B.n_fine_max, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap, B.fine_lds);
   DevNsx n_arg = N;
   BatchLaunch b_arg = B;
   void* args[] = {&n_arg, &b_arg};
   const hipError_t e =
-      hipLaunchKernel(batched_fn(B.atmosphere, B.epl, B.emode), dim3(n_wg), dim3(kBThreads), args, lds, st);
+      hipLaunchKernel(batched_fn(B.atmosphere, B.epl, B.emode, B.fine_lds), dim3(n_wg), dim3(kBThreads), args, lds, st);
   return e != hipSuccess ? e : hipGetLastError();
 }
 
@@ -620,10 +627,10 @@ int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int 
   return hipGetLastError();
 }
 
-int batched_occupancy(int atmosphere, int epl, int emode, size_t lds, int* wg_per_cu) {
+int batched_occupancy(int atmosphere, int epl, int emode, bool fine_lds, size_t lds, int* wg_per_cu) {
   int n = 0;
   const hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl, emode), kBThreads, lds);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl, emode, fine_lds), kBThreads, lds);
   *wg_per_cu = n;
   return e;
 }
