@@ -45,8 +45,10 @@ struct BatchItem {
   int e0;      // first energy of the chunk
   int grid;    // index into BatchLaunch::grids
   int en;      // energies in the chunk (its accumulator row length)
+  int c0, c1;  // phase classes [c0, c1) of each ring (a ring split by class is one item per range)
   size_t acc;  // first double of the item's accumulator copies, [copies][n_phase][en]
 };
+constexpr int kAllClasses = 1 << 30;
 
 struct BatchLaunch {
   const SpotParams* spots;

@@ -249,7 +249,8 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const double c_shift = point ? Dp->ps_shift : 1.0 - 0.25 * Dp->s_shift2;
       const int n_words = (n + 63) / 64 * 2;  // coverage words, 64-position batches
 
-      for (int ci = 0; ci < cset.n_classes; ++ci) {
+      const int c_end = min(It.c1, cset.n_classes);
+      for (int ci = It.c0; ci < c_end; ++ci) {
         const ClassHdr C = G.chdr[cset.first_class + ci];
         const double c_o = C.delta + c_shift;
         // rotational phase of position k, wrapped into [ph[0], ph[0] + 1)
