@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 METRIC = "profile evals/s, batched vs plain GPU, at ≤1e-4 max rel err; p50/p99 latency"
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak
 FLOP_PER_EVAL = 32         # 4 x 4 (logEkT x mu) Lagrange stencil = 16 FMA
+SERVE_FIXED_RATE = 37.0    # requests/s of the fixed-load serving stream
 N_SIDE, N_PHASE, N_E = 1024, 64, 360
 
 
@@ -297,7 +298,9 @@ def bench_serving(root, duration_s, burst, seed=7):
     {256, 1024, 2048} x {32, 64, 128} bins (2 GL nodes) x {128, 360}
     energies (folded through the 300 x 128 or 300 x 360 response).  The
     capacity is first measured on a burst of `burst` requests submitted at
-    once; the arrival rate is then 80 % of it for `duration_s` seconds.
+    once; the arrival rate is then 80 % of it for `duration_s` seconds, and
+    a second stream arrives at the fixed rate SERVE_FIXED_RATE (latency at a
+    load that does not move with the capacity).
     Latency is submit -> result on the host, reported by ppm_poll.  Every
     100th request of the timed stream is audited afterwards against a direct
     plain-path evaluation of the same star and grid, and against the FP64
@@ -341,13 +344,24 @@ def bench_serving(root, duration_s, burst, seed=7):
             time.sleep(0.0005)
         return time.perf_counter() - t0, np.array(lat)
 
+    def stream(rate, n_req):
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        reqs = draw(n_req)
+        return reqs, arrivals
+
     run(draw(len(combos)), None)  # warm-up: every grid once
     t_burst, _ = run(draw(burst), None)
     capacity = burst / t_burst
     rate = 0.8 * capacity
+    n_fixed = max(1, int(SERVE_FIXED_RATE * duration_s))
+    reqs_f, arr_f = stream(SERVE_FIXED_RATE, n_fixed)
+    log("bench: serving %d requests at a fixed %.1f/s (capacity %.1f/s)" % (n_fixed, SERVE_FIXED_RATE, capacity))
+    dt_f, lat_f = run(reqs_f, arr_f)
+    fixed = {"requests": n_fixed, "offered_per_s": SERVE_FIXED_RATE, "completed_per_s": n_fixed / dt_f,
+             "p50_ms": float(np.percentile(lat_f, 50)), "p99_ms": float(np.percentile(lat_f, 99)),
+             "max_ms": float(lat_f.max())}
     n_req = max(1, int(rate * duration_s))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    reqs = draw(n_req)
+    reqs, arrivals = stream(rate, n_req)
     log("bench: serving %d requests at %.1f/s (capacity %.1f/s)" % (n_req, rate, capacity))
     audit = {}
     s1 = ctx.serve_stats()
@@ -394,10 +408,10 @@ def bench_serving(root, duration_s, burst, seed=7):
             "completed_per_s": n_req / dt, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
             "evals_per_s_ref": ref_evals / dt, "audit_max_rel_err_vs_plain": err,
-            "audit_max_rel_err_vs_fp64_oracle": err_o, "lanes": lanes,
+            "audit_max_rel_err_vs_fp64_oracle": err_o, "lanes": lanes, "fixed_rate": fixed,
             "workload": "prior random walk (2%% steps), N_side {256,1024,2048} x bins {32,64,128} x 2 GL "
-                        "x E {128,360}, open-loop Poisson arrivals at 80%% of burst capacity for %.0f s"
-                        % duration_s}
+                        "x E {128,360}, open-loop Poisson arrivals at 80%% of burst capacity for %.0f s; "
+                        "fixed_rate: the same at %.1f/s" % (duration_s, SERVE_FIXED_RATE)}
 
 
 def copy_calibration(mib):
