@@ -537,27 +537,36 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
 }
 
 // flux[s - s0][b][e] = sum over the star's items whose energy chunk holds e
-// (in item order) of scale(spot) * (sum of the item's accumulator copies) /
-// E, for the stars [s0, s1) of one grid.
-__global__ void k_reduce_batched(BatchLaunch B, int s0, int s1, int n_phase, int n_e, const double* energies,
-                                 double* flux) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per = (size_t)n_phase * n_e;
-  if (idx >= per * (s1 - s0)) return;
-  const int s = s0 + (int)(idx / per);
-  const size_t o = idx - (size_t)(s - s0) * per;
-  const int b = (int)(o / n_e), e = (int)(o - (size_t)b * n_e);
+// of scale(spot) * (sum of the item's accumulator copies) / E, for the stars
+// [s0, s1) of one grid.  One workgroup per (64 energies, bin, star); wave w
+// sums the items w, w + 4, ... of the star in item order and wave 0 adds the
+// four partial sums in wave order, so the result does not depend on timing.
+constexpr int kReduceWaves = 4;
+__global__ __launch_bounds__(64 * kReduceWaves) void k_reduce_batched(BatchLaunch B, int s0, int n_phase, int n_e,
+                                                                      const double* energies, double* flux) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane, b = blockIdx.y, s = s0 + (int)blockIdx.z;
+  __shared__ double part[kReduceWaves][64];
   double total = 0.0;
-  for (int it = B.star_item0[s]; it < B.star_item0[s + 1]; ++it) {
-    const BatchItem I = B.items[it];
-    if (e < I.e0 || e >= I.e0 + I.en) continue;  // another chunk's item: it wrote none of e
-    const size_t cp = (size_t)n_phase * I.en;
-    const size_t oi = I.acc + (size_t)b * I.en + (e - I.e0);
-    double a = 0.0;
-    for (int c = 0; c < kBWP; ++c) a += B.acc[oi + c * cp];
-    total += B.spots[I.spot].scale * a;
+  if (e < n_e) {
+    for (int it = B.star_item0[s] + wave; it < B.star_item0[s + 1]; it += kReduceWaves) {
+      const BatchItem I = B.items[it];
+      if (e < I.e0 || e >= I.e0 + I.en) continue;  // another chunk's item: it wrote none of e
+      const size_t cp = (size_t)n_phase * I.en;
+      const double* a = B.acc + I.acc + (size_t)b * I.en + (e - I.e0);
+      double sum = 0.0;
+#pragma unroll
+      for (int c = 0; c < kBWP; ++c) sum += a[c * cp];
+      total += B.spots[I.spot].scale * sum;
+    }
   }
-  flux[idx] = total / energies[e];
+  part[wave][lane] = total;
+  __syncthreads();
+  if (wave == 0 && e < n_e) {
+    double f = part[0][lane];
+    for (int w = 1; w < kReduceWaves; ++w) f += part[w][lane];
+    flux[((size_t)(s - s0) * n_phase + b) * n_e + e] = f / energies[e];
+  }
 }
 
 }  // namespace
@@ -621,11 +630,17 @@ int launch_zero_u64(unsigned long long* p, int n, void* stream) {
 }
 
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream) {
-  const size_t n = (size_t)G.n_phase * G.n_e * (s1 - s0);
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, B, s0, s1,
-                     G.n_phase, G.n_e, G.energies, flux);
-  return hipGetLastError();
+  if (s1 <= s0 || G.n_phase < 1 || G.n_e < 1) return hipSuccess;
+  constexpr int kMaxZ = 65535;  // grid z limit: stars per launch
+  for (int a = s0; a < s1; a += kMaxZ) {
+    const int n = min(kMaxZ, s1 - a);
+    const dim3 grid((unsigned)((G.n_e + 63) / 64), (unsigned)G.n_phase, (unsigned)n);
+    hipLaunchKernelGGL(k_reduce_batched, grid, dim3(64 * kReduceWaves), 0, (hipStream_t)stream, B, a, G.n_phase,
+                       G.n_e, G.energies, flux + (size_t)(a - s0) * G.n_phase * G.n_e);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 int batched_occupancy(int atmosphere, int epl, int emode, bool fine_lds, size_t lds, int* wg_per_cu) {
