@@ -158,3 +158,29 @@ def test_auto_method_for_one_star(ctx):
         assert np.array_equal(fa, fe)
         assert sa.n_eval_done == se.n_eval_done
         assert (sa.n_eval_done < sa.n_eval_ref) == (expect == abi.METHOD_BATCHED)
+
+
+def test_batched_empty_and_hidden_spots(ctx, orc):
+    """One batched launch holding a spot smaller than a pixel (no active
+    patch), a spot the observer never sees (at the south pole, observer 3 deg
+    from the north pole) and an ordinary star: each equals the oracle (the
+    first two are zero), and a launch of empty spots alone returns zeros."""
+    g = batch_grid(32, 16)
+    kw = dict(atmosphere=abi.ATM_NSX, variant=abi.VARIANT_DEVARSHI)
+    empty = abi.make_star([abi.make_spot(1.0, 1e-4, 0.3)], **kw)
+    hidden = abi.make_star([abi.make_spot(np.pi, 0.3, 0.3)], inclination_rad=0.05, **kw)
+    (normal,) = S.sample_priors(1, seed=8)
+    stars = [empty, hidden, normal]
+    print("case: batched empty + hidden + ordinary spot, N 32", flush=True)
+    flux, counts, stats = ctx.eval(stars, g)
+    assert np.isfinite(flux).all() and np.isfinite(counts).all()
+    for k, st in enumerate(stars):
+        ref, info = orc.profile(st, g, threads=16)
+        if k < 2:
+            assert (info.n_active > 0) == (k == 1)  # the hidden spot has patches, none seen
+            assert np.all(ref == 0) and np.all(flux[k] == 0) and np.all(counts[k] == 0), k
+        else:
+            assert rel_err(flux[k], ref) <= TOL
+    assert stats.n_active > 0
+    f0, c0, s0 = ctx.eval([empty, empty], g)
+    assert s0.n_active == 0 and np.all(f0 == 0) and np.all(c0 == 0)
