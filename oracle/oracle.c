@@ -12,6 +12,16 @@
  *   - the emitted intensity is divided by E_obs (cpu/os1_cpu.cpp:180);
  *   - patch membership and visibility are FP64.
  * Each function cites the reference lines it follows.
+ *
+ * Pinning (no function is left unpinned): or_profile and or_fold against the
+ * outputs of the reference's own os1_cpu, devarshi24st_cpu and t_interp
+ * built from their sources (tests/test_oracle_golden.py, <= 5e-10, with a
+ * planted error that the check catches), which also pins the lensing
+ * look-ups and the active counts they go through; or_vec_hunt,
+ * or_gl_node/weight, or_healpix_ring/phi and or_nsx_interp (all five
+ * interpolants, clamps included) in addition by brute force, closed forms
+ * and exactness properties (tests/test_oracle_math.py); the NSX path against
+ * the blackbody closed form (tests/test_blackbody_pin.py).
  */
 #define _GNU_SOURCE
 #include "oracle.h"
