@@ -539,9 +539,11 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
 // flux[s - s0][b][e] = sum over the star's items whose energy chunk holds e
 // of scale(spot) * (sum of the item's accumulator copies) / E, for the stars
 // [s0, s1) of one grid.  One workgroup per (64 energies, bin, star); wave w
-// sums the items w, w + 4, ... of the star in item order and wave 0 adds the
-// four partial sums in wave order, so the result does not depend on timing.
-constexpr int kReduceWaves = 4;
+// sums the items w, w + 16, ... of the star in item order and wave 0 adds
+// the 16 partial sums in wave order, so the result does not depend on timing
+// (16 waves: one star's ~1500 accumulators of split rings are read in 0.5 ms
+// against 1.0 with 4).
+constexpr int kReduceWaves = 16;
 __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce_batched(BatchLaunch B, int s0, int n_phase, int n_e,
                                                                       const double* energies, double* flux) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
