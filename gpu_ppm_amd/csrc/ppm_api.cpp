@@ -499,12 +499,12 @@ ppm_status eval_plain_rows(ppm_ctx* c, Lane& L, const ppm_star& st, const ppm_gr
 // PPM_METHOD_AUTO for one star: the batched path (phase-shift dedupe, one
 // star per launch) where it is the faster one on MI355X and can take the
 // grid, else the plain path.  Measured per profile (64 bins x 2 GL, 360
-// energies): N_side 128 plain 1.5 ms / batched 2.1 ms, 256 4.0 / 3.6, 1024
-// 40 / 11, 2048 152 / 27; with 32 bins and 128 energies the batched path is
-// ahead from N_side 128 (1.2 / 1.0 ms).
+// energies): N_side 64 plain 0.9 ms / batched 1.0 ms, 128 1.6 / 1.5, 1024
+// 40 / 7.9, 2048 153 / 22; with 32 bins and 128 energies the batched path is
+// ahead from N_side 32 (1.2 / 0.3 ms).
 static int auto_single_method(const ppm_ctx* c, const ppm_star& star, const ppm_grid& g,
                               const std::vector<double>& energies) {
-  const int n_side_min = energies.size() <= 128 ? 128 : 256;
+  const int n_side_min = energies.size() <= 128 ? 32 : 128;
   if (g.n_side < n_side_min) return PPM_METHOD_PLAIN;
   for (size_t q = 1; q < energies.size(); ++q)
     if (!(energies[q] > energies[q - 1])) return PPM_METHOD_PLAIN;

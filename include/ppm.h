@@ -85,7 +85,7 @@ enum { PPM_SAMPLE_POINTS = 0, PPM_SAMPLE_GL = 1 };
 /* Evaluation method. */
 enum {
   PPM_METHOD_AUTO = 0,    /* BATCHED for several stars; for one star BATCHED
-                           * from N_side 256 (128 with <= 128 energies) when
+                           * from N_side 128 (32 with <= 128 energies) when
                            * its energies ascend and it fits the LDS, else PLAIN */
   PPM_METHOD_PLAIN = 1,   /* reference iteration space, one profile per launch */
   PPM_METHOD_BATCHED = 2  /* persistent work queue + phase-shift dedupe        */

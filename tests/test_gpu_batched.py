@@ -145,9 +145,10 @@ def test_energy_chunks_after_a_larger_batch(ctx, orc):
 
 
 def test_auto_method_for_one_star(ctx):
-    """PPM_METHOD_AUTO with one star takes the batched path from N_side 256
-    (ascending energies), the plain path below it and for unordered energies:
-    the results equal those of the method named explicitly."""
+    """PPM_METHOD_AUTO with one star (360 energies) takes the batched path
+    from N_side 128 (ascending energies), the plain path below it and for
+    unordered energies: the results equal those of the method named
+    explicitly."""
     (star,) = S.sample_priors(1, seed=4)
     cases = [(256, dict(response=0), abi.METHOD_BATCHED), (64, dict(response=0), abi.METHOD_PLAIN),
              (256, dict(response=-1, energies=[2.0, 1.0, 3.0]), abi.METHOD_PLAIN)]
