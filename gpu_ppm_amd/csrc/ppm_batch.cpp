@@ -649,6 +649,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   B.acc = (double*)LB->d_acc.p;
   B.seg = (double*)LB->d_seg.p;
   B.n_items = n_items;
+  B.n_stars = n;
   B.n_fine_max = nf_max;
   B.nsx_rows = nsx_rows;
   B.cov_words = cov_words;

@@ -63,6 +63,7 @@ struct BatchLaunch {
   double* acc;             // item accumulators
   double* seg;             // [n_wg][segment totals], workgroup scratch
   int n_items;
+  int n_stars;             // stars of the launch (all grids)
   int n_fine_max;          // LDS layout: the largest grid of the launch
   int nsx_rows, cov_words, ev_cap, atmosphere;
   int epl;                 // energies per lane: 6, or 2 when every grid has <= 128 energies

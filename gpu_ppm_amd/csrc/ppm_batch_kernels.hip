@@ -538,20 +538,21 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
 
 // flux[s - s0][b][e] = sum over the star's items whose energy chunk holds e
 // of scale(spot) * (sum of the item's accumulator copies) / E, for the stars
-// [s0, s1) of one grid.  One workgroup per (64 energies, bin, star); wave w
-// sums the items w, w + 16, ... of the star in item order and wave 0 adds
-// the 16 partial sums in wave order, so the result does not depend on timing
-// (16 waves: one star's ~1500 accumulators of split rings are read in 0.5 ms
-// against 1.0 with 4).
-constexpr int kReduceWaves = 16;
-__global__ __launch_bounds__(64 * kReduceWaves) void k_reduce_batched(BatchLaunch B, int s0, int n_phase, int n_e,
-                                                                      const double* energies, double* flux) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// [s0, s1) of one grid.  One workgroup per (64 energies, bin, star) of
+// n_waves = blockDim.x / 64 waves: wave w sums the items w, w + n_waves, ...
+// of the star in item order and wave 0 adds the partial sums in wave order,
+// so the result does not depend on timing.  The host takes 16 waves when the
+// stars have many items (a star alone, its rings split by phase class: ~1500
+// accumulators read in 0.5 ms against 1.0 with 4) and 4 otherwise.
+constexpr int kReduceWavesMax = 16;
+__global__ __launch_bounds__(64 * kReduceWavesMax) void k_reduce_batched(BatchLaunch B, int s0, int n_phase, int n_e,
+                                                                         const double* energies, double* flux) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
   const int e = blockIdx.x * 64 + lane, b = blockIdx.y, s = s0 + (int)blockIdx.z;
-  __shared__ double part[kReduceWaves][64];
+  __shared__ double part[kReduceWavesMax][64];
   double total = 0.0;
   if (e < n_e) {
-    for (int it = B.star_item0[s] + wave; it < B.star_item0[s + 1]; it += kReduceWaves) {
+    for (int it = B.star_item0[s] + wave; it < B.star_item0[s + 1]; it += n_waves) {
       const BatchItem I = B.items[it];
       if (e < I.e0 || e >= I.e0 + I.en) continue;  // another chunk's item: it wrote none of e
       const size_t cp = (size_t)n_phase * I.en;
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce_batched(BatchLaunc
   __syncthreads();
   if (wave == 0 && e < n_e) {
     double f = part[0][lane];
-    for (int w = 1; w < kReduceWaves; ++w) f += part[w][lane];
+    for (int w = 1; w < n_waves; ++w) f += part[w][lane];
     flux[((size_t)(s - s0) * n_phase + b) * n_e + e] = f / energies[e];
   }
 }
@@ -633,11 +634,12 @@ int launch_zero_u64(unsigned long long* p, int n, void* stream) {
 
 int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int s1, double* flux, void* stream) {
   if (s1 <= s0 || G.n_phase < 1 || G.n_e < 1) return hipSuccess;
+  const int waves = B.n_items >= 64 * B.n_stars ? kReduceWavesMax : 4;
   constexpr int kMaxZ = 65535;  // grid z limit: stars per launch
   for (int a = s0; a < s1; a += kMaxZ) {
     const int n = min(kMaxZ, s1 - a);
     const dim3 grid((unsigned)((G.n_e + 63) / 64), (unsigned)G.n_phase, (unsigned)n);
-    hipLaunchKernelGGL(k_reduce_batched, grid, dim3(64 * kReduceWaves), 0, (hipStream_t)stream, B, a, G.n_phase,
+    hipLaunchKernelGGL(k_reduce_batched, grid, dim3(64 * waves), 0, (hipStream_t)stream, B, a, G.n_phase,
                        G.n_e, G.energies, flux + (size_t)(a - s0) * G.n_phase * G.n_e);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
