@@ -225,7 +225,7 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
         log("bench: autotuning the batched launch shape on the %d stars" % n_stars)
         tr = ctx.tune(stars, gb, repeats=1)
         shape = {"items_per_slot": tr.best.items_per_slot, "wg_per_cu": tr.best.wg_per_cu,
-                 "tuned_integral_ms": tr.best_ms, "default_integral_ms": tr.default_ms, "shapes_tried": tr.n_tried}
+                 "energies_per_lane": tr.best.energies_per_lane, "tuned_integral_ms": tr.best_ms, "default_integral_ms": tr.default_ms, "shapes_tried": tr.n_tried}
         log("bench: tuned shape %s" % shape)
     log("bench: batched %d stars, warmup %d, steps %d" % (n_stars, warmup, steps))
     dt, st = timed(ctx, stars, gb, d_flux, d_cnt, steps, warmup)
@@ -235,7 +235,8 @@ def bench_batched(root, steps, warmup, n_stars, plain_stars, tune=True):
     os.environ["PPM_NO_GRAPH"] = "1"
     ctx = Context(root, lensing="wide", responses=("std",))
     if shape:
-        ctx.set_launch_shape(gb, n_stars, abi.ATM_NSX, shape["items_per_slot"], shape["wg_per_cu"])
+        ctx.set_launch_shape(gb, n_stars, abi.ATM_NSX, shape["items_per_slot"], shape["wg_per_cu"],
+                             shape["energies_per_lane"])
     _, sk = timed(ctx, stars, gb, d_flux, d_cnt, 1, 0)
     # plain path on the same stars, one profile per launch sequence
     gp = grid(abi.METHOD_PLAIN)

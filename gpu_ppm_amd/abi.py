@@ -102,7 +102,7 @@ class Fit(ctypes.Structure):
 
 
 class LaunchShape(ctypes.Structure):
-    _fields_ = [("items_per_slot", c_int), ("wg_per_cu", c_int)]
+    _fields_ = [("items_per_slot", c_int), ("wg_per_cu", c_int), ("energies_per_lane", c_int), ("reserved", c_int)]
 
 
 class TuneResult(ctypes.Structure):

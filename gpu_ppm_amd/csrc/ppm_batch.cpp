@@ -294,22 +294,48 @@ static ppm_status class_tables(ppm_ctx* c, BatchState* BS, const ppm_grid& g, co
 
 constexpr int kEvCap = 512;  // events per class group in LDS
 
-// LDS layout of a launch over `groups`: the largest fine table, slice rows
-// and coverage bitmap among them.  Returns the LDS bytes per workgroup.
-// Energies per lane of a launch: 2 when every grid has at most 128
-// energies, else 6.
-static int launch_epl(const std::vector<BatchGroup>& groups) {
+// True when, at epl energies per lane, every row set of a work item (epl/2 x
+// 64 consecutive energies of a 64 epl-energy chunk) spans at most the slice
+// rows the table evaluation takes (58: WaveEnergies::fast; 57 here, below
+// the device's FP32 rounding).  Beyond it a wavefront evaluates per energy.
+static bool sets_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups, int epl) {
+  const double h = (c->N.logE_max - c->N.logE_min) / std::max(c->N.nE - 1, 1);
+  const int set = 32 * epl, chunk = 64 * epl;
+  for (const BatchGroup& G : groups) {
+    const int n = (int)G.energies.size();
+    for (int a = 0; a < n; a += set) {
+      const int b = std::min(a + set, std::min(n, (a / chunk + 1) * chunk)) - 1;
+      if (std::log10(G.energies[b] / G.energies[a]) / h > 57.0) return false;
+    }
+  }
+  return true;
+}
+
+// Energies per lane of a launch: the fewest of 2, 4 or 6 whose work item
+// holds every grid's energies in one chunk, if its row sets fit (NSX); else
+// the most whose row sets fit, in several chunks per spot; 2 at least.  A
+// 256-energy grid over 0.2-12 keV takes 4 (45 rows per set; at 6, 67 rows
+// per set would fall back to per-energy evaluation, 2.5x slower).
+static int launch_epl(const ppm_ctx* c, const std::vector<BatchGroup>& groups, int atm) {
   size_t ne = 0;
   for (const BatchGroup& G : groups) ne = std::max(ne, G.energies.size());
-  return ne <= 128 ? 2 : 6;
+  for (int epl : {2, 4, 6})
+    if (ne <= (size_t)(64 * epl)) {
+      if (atm != PPM_ATM_NSX || sets_fit(c, groups, epl)) return epl;
+      break;
+    }
+  if (atm != PPM_ATM_NSX) return 6;
+  for (int epl : {6, 4})
+    if (sets_fit(c, groups, epl)) return epl;
+  return 2;
 }
 
 // LDS layout of a launch over `groups`: the largest fine table, slice rows
 // and coverage bitmap among them.  Returns the LDS bytes per workgroup with
 // the fine tables staged in LDS (fine_lds) or read from global memory.
 static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& groups, int atm, int rows_override,
-                              bool fine_lds, int& nsx_rows, int& nf_max, int& cov_words) {
-  const int per_item = batch_energies(launch_epl(groups));
+                              int epl, bool fine_lds, int& nsx_rows, int& nf_max, int& cov_words) {
+  const int per_item = batch_energies(epl);
   nsx_rows = nf_max = cov_words = 0;
   const double h = (c->N.logE_max - c->N.logE_min) / std::max(c->N.nE - 1, 1);
   for (const BatchGroup& G : groups) {
@@ -332,10 +358,16 @@ static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& g
   return batch_lds_bytes(nf_max, atm, nsx_rows, cov_words, kEvCap, fine_lds);
 }
 
+int batch_grid_epl(const ppm_ctx* c, const BatchGroup& G) {
+  return launch_epl(c, {G}, G.stars[0].atmosphere);
+}
+
 bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups) {
   if (groups.empty() || !groups[0].n) return false;
   int r, f, w;
-  return launch_geometry(c, groups, groups[0].stars[0].atmosphere, 0, false, r, f, w) <= (size_t)c->max_lds;
+  const int atm = groups[0].stars[0].atmosphere;
+  return launch_geometry(c, groups, atm, 0, launch_epl(c, groups, atm), false, r, f, w) <=
+         (size_t)c->max_lds;
 }
 
 static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& groups, ppm_stats* stats,
@@ -358,7 +390,19 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
         return fail(c, PPM_ERR_ARG, "a launch needs one atmosphere and one energy-axis interpolant");
   }
   int e;
-  const int epl = launch_epl(groups);
+  // the launch shape: forced (autotuner), stored for a one-grid launch, or
+  // the defaults
+  ppm_launch_shape shape{0, 0, 0, 0};
+  if (L.forced) {
+    shape = *L.forced;
+  } else if (n_groups == 1) {
+    std::lock_guard<std::mutex> lk(BS->shape_mu);
+    auto it = BS->shapes.find(shape_key(groups[0].g, (int)groups[0].energies.size(), atm, n));
+    if (it != BS->shapes.end()) shape = it->second;
+  }
+  const int epl = shape.energies_per_lane == 2 || shape.energies_per_lane == 4 || shape.energies_per_lane == 6
+                      ? shape.energies_per_lane
+                      : launch_epl(c, groups, atm);
   const int per_item = batch_energies(epl);
 
   // ---- class tables (per grid), held in use until this launch has ended
@@ -436,8 +480,8 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   // ---- launch geometry: the LDS layout of the largest grid
   // the fine tables go to LDS unless that lowers the occupancy
   int nsx_rows, nf_max, cov_words;
-  const size_t lds_global = launch_geometry(c, groups, atm, rows_override, false, nsx_rows, nf_max, cov_words);
-  const size_t lds_staged = launch_geometry(c, groups, atm, rows_override, true, nsx_rows, nf_max, cov_words);
+  const size_t lds_global = launch_geometry(c, groups, atm, rows_override, epl, false, nsx_rows, nf_max, cov_words);
+  const size_t lds_staged = launch_geometry(c, groups, atm, rows_override, epl, true, nsx_rows, nf_max, cov_words);
   if (lds_global > (size_t)c->max_lds) return fail(c, PPM_ERR_ARG, "grid too large for the batched kernel's LDS");
   int occ_global = 0, occ_staged = 0;
   if ((e = batched_prepare(atm, epl, emode, false, c->max_lds))) return hip_fail(c, e, "kernel attributes");
@@ -450,14 +494,6 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   const size_t lds = fine_lds ? lds_staged : lds_global;
   int wg_per_cu = fine_lds ? occ_staged : occ_global;
   L.occ_wg_max = wg_per_cu;
-  ppm_launch_shape shape{0, 0};
-  if (L.forced) {
-    shape = *L.forced;
-  } else if (n_groups == 1) {
-    std::lock_guard<std::mutex> lk(BS->shape_mu);
-    auto it = BS->shapes.find(shape_key(groups[0].g, (int)groups[0].energies.size(), atm, n));
-    if (it != BS->shapes.end()) shape = it->second;
-  }
   if (shape.wg_per_cu > 0) wg_per_cu = std::min(wg_per_cu, shape.wg_per_cu);
   const int slots = wg_per_cu * c->n_cu;
 
@@ -828,7 +864,7 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   }
   BatchState* BS = state(c);
   Lane& L = c->main;
-  ppm_launch_shape cur{0, 0};
+  ppm_launch_shape cur{0, 0, 0, 0};
   // direct launches (the integral's own events) and the candidate shape,
   // restored on every exit
   struct Restore {
@@ -860,12 +896,12 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   // untimed warm-up (allocations, first launches); it also gives the
   // occupancy limit
   {
-    cur = ppm_launch_shape{0, 0};
+    cur = ppm_launch_shape{0, 0, 0, 0};
     ppm_stats st{};
     if ((s = eval_any(c, L, stars, n, &g, (double*)d_flux.p, nullptr, &st))) return s;
   }
   const int wg_max = std::max(1, L.occ_wg_max);
-  ppm_launch_shape best{kDefaultItemsPerSlot, wg_max};
+  ppm_launch_shape best{kDefaultItemsPerSlot, wg_max, 0, 0};
   double t_best, t_def;
   if ((s = time_shape(best, t_def))) return s;
   t_best = t_def;
@@ -881,13 +917,29 @@ ppm_status ppm_tune(ppm_ctx* c, const ppm_star* stars, int n, const ppm_grid* gr
   };
   // items per workgroup: halve and double, then one more step the better way
   for (int f : {kDefaultItemsPerSlot / 2, kDefaultItemsPerSlot * 2})
-    if ((s = consider(ppm_launch_shape{f, wg_max}))) return s;
+    if ((s = consider(ppm_launch_shape{f, wg_max, 0, 0}))) return s;
   if (best.items_per_slot != kDefaultItemsPerSlot) {
     const int f = best.items_per_slot < kDefaultItemsPerSlot ? best.items_per_slot / 2 : best.items_per_slot * 2;
-    if ((s = consider(ppm_launch_shape{std::max(f, 1), wg_max}))) return s;
+    if ((s = consider(ppm_launch_shape{std::max(f, 1), wg_max, 0, 0}))) return s;
   }
   // one resident workgroup fewer per CU
-  if (wg_max > 1 && (s = consider(ppm_launch_shape{best.items_per_slot, wg_max - 1}))) return s;
+  if (wg_max > 1 && (s = consider(ppm_launch_shape{best.items_per_slot, wg_max - 1, 0, 0}))) return s;
+  // more than 128 energies: the other energies-per-lane choices whose row
+  // sets fit (fewer per lane: more energy chunks, each patch set up again;
+  // more: idle lanes in the last chunk)
+  if (key[5] > 128) {
+    BatchGroup G;
+    G.stars = stars;
+    G.n = n;
+    G.g = g;
+    G.energies = energies;
+    const int dflt = launch_epl(c, {G}, stars[0].atmosphere);
+    const ppm_launch_shape base = best;
+    for (int epl : {2, 4, 6})
+      if (epl != dflt && (stars[0].atmosphere != PPM_ATM_NSX || sets_fit(c, {G}, epl)) &&
+          (s = consider(ppm_launch_shape{base.items_per_slot, base.wg_per_cu, epl, 0})))
+        return s;
+  }
   {
     std::lock_guard<std::mutex> lk(BS->shape_mu);
     BS->shapes[key] = best;
@@ -904,18 +956,24 @@ ppm_status ppm_set_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, i
                                 const ppm_launch_shape* shape) {
   if (!c) return PPM_ERR_ARG;
   if (shape && (shape->items_per_slot < 0 || shape->items_per_slot > 4096 || shape->wg_per_cu < 0 ||
-                shape->wg_per_cu > 64))
-    return fail(c, PPM_ERR_ARG, "launch shape out of range (items_per_slot 0..4096, wg_per_cu 0..64)");
+                shape->wg_per_cu > 64 ||
+                (shape->energies_per_lane != 0 && shape->energies_per_lane != 2 && shape->energies_per_lane != 4 &&
+                 shape->energies_per_lane != 6)))
+    return fail(c, PPM_ERR_ARG,
+                "launch shape out of range (items_per_slot 0..4096, wg_per_cu 0..64, energies_per_lane 0, 2, 4 or 6)");
   std::lock_guard<std::mutex> lk(c->exec_mu);
   std::vector<int> key;
   ppm_status s = shape_args(c, grid, n_stars, atmosphere, key);
   if (s) return s;
   BatchState* BS = state(c);
   std::lock_guard<std::mutex> lk2(BS->shape_mu);
-  if (shape && (shape->items_per_slot || shape->wg_per_cu))
-    BS->shapes[key] = *shape;
-  else
+  if (shape && (shape->items_per_slot || shape->wg_per_cu || shape->energies_per_lane)) {
+    ppm_launch_shape sh = *shape;
+    sh.reserved = 0;
+    BS->shapes[key] = sh;
+  } else {
     BS->shapes.erase(key);
+  }
   return PPM_OK;
 }
 
@@ -929,7 +987,7 @@ ppm_status ppm_get_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, i
   BatchState* BS = state(c);
   std::lock_guard<std::mutex> lk2(BS->shape_mu);
   auto it = BS->shapes.find(key);
-  *shape = it != BS->shapes.end() ? it->second : ppm_launch_shape{0, 0};
+  *shape = it != BS->shapes.end() ? it->second : ppm_launch_shape{0, 0, 0, 0};
   return PPM_OK;
 }
 

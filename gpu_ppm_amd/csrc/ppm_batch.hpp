@@ -37,5 +37,7 @@ ppm_status eval_batched_groups(ppm_ctx* c, Lane& L, std::vector<BatchGroup>& gro
 
 // Whether one launch over these groups fits the LDS of a workgroup.
 bool batch_groups_fit(const ppm_ctx* c, const std::vector<BatchGroup>& groups);
+// Energies per lane the batched kernel takes for this grid alone (2 or 6).
+int batch_grid_epl(const ppm_ctx* c, const BatchGroup& G);
 
 }  // namespace ppm

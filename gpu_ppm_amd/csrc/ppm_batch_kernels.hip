@@ -27,8 +27,9 @@ namespace {
 constexpr int kBW = 4;                  // wavefronts per workgroup, each a segment of the positions
 constexpr int kBWP = kBW;
 constexpr int kBThreads = 64 * kBW;
-// Energies per lane: 6 (384 per item, energy grids beyond 128 points) or
-// 2 (128 per item); a lane's energies form two row sets either way.
+// Energies per lane: 2, 4 or 6 (128, 256 or 384 energies per work item, the
+// host's choice by energy count and row span); a lane's energies form two
+// row sets in every case.
 constexpr int kMaxEPL = 6;
 constexpr int kBSets = 2;
 
@@ -586,7 +587,11 @@ int batch_energies(int epl) { return 64 * epl; }
 namespace {
 template <int ATM, int EM, bool FL>
 const void* batched_fn_epl(int epl) {
-  return epl == 2 ? (const void*)k_integral_batched<ATM, 2, EM, FL> : (const void*)k_integral_batched<ATM, 6, EM, FL>;
+  switch (epl) {
+    case 2: return (const void*)k_integral_batched<ATM, 2, EM, FL>;
+    case 4: return (const void*)k_integral_batched<ATM, 4, EM, FL>;
+    default: return (const void*)k_integral_batched<ATM, 6, EM, FL>;
+  }
 }
 template <int ATM, int EM>
 const void* batched_fn_fl(int epl, bool fine_lds) {

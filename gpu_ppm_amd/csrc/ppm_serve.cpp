@@ -244,20 +244,20 @@ void worker_loop(ppm_ctx* c, int w) {
         G.d_flux = (double*)W.d_flux.p + f_off[gi];
         G.d_counts = r0.n_ch ? (double*)W.d_counts.p + c_off[gi] : nullptr;
       }
-      // launches: one per class of grids (at most 128 energies or more,
-      // fine tables up to 384 nodes or more: the kernel instance and the
-      // LDS, hence the occupancy, follow the largest grid of a launch),
+      // launches: one per class of grids (2, 4 or 6 energies per lane, fine
+      // tables up to 384 nodes or more: the kernel instance and the LDS,
+      // hence the occupancy, follow the largest grid of a launch),
       // groups packed in order while the LDS of one workgroup holds them;
       // a group the batched kernel cannot take (energies not strictly
       // increasing, or too large alone) runs on the plain path
       std::vector<std::vector<int>> launches;
       std::vector<int> plain;
-      for (int cls = 0; cls < 4; ++cls) {
+      for (int cls = 0; cls < 6; ++cls) {  // (2, 4 or 6 energies per lane) x (n_fine <= 384 or more)
         std::vector<BatchGroup> cur;
         std::vector<int> cur_idx;
         for (int gi = 0; gi < n_groups; ++gi) {
           const BatchGroup& G = groups[gi];
-          if (((G.energies.size() > 128) | (G.g.n_fine > 384) << 1) != cls) continue;
+          if ((batch_grid_epl(c, G) / 2 - 1) + 3 * (G.g.n_fine > 384) != cls) continue;
           bool ascending = true;
           for (size_t q = 1; q < G.energies.size(); ++q) ascending = ascending && G.energies[q] > G.energies[q - 1];
           if (!ascending || !batch_groups_fit(c, {G})) {

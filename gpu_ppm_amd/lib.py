@@ -269,15 +269,16 @@ class Context:
         self._check(load().ppm_tune(self._h, arr, len(arr), byref(grid), repeats, byref(r)))
         return r
 
-    def set_launch_shape(self, grid, n_stars, atmosphere, items_per_slot=0, wg_per_cu=0):
-        sh = abi.LaunchShape(items_per_slot, wg_per_cu)
+    def set_launch_shape(self, grid, n_stars, atmosphere, items_per_slot=0, wg_per_cu=0, energies_per_lane=0):
+        sh = abi.LaunchShape(items_per_slot, wg_per_cu, energies_per_lane, 0)
         self._check(load().ppm_set_launch_shape(self._h, byref(grid), n_stars, atmosphere, byref(sh)))
 
     def launch_shape(self, grid, n_stars, atmosphere):
-        """(items_per_slot, wg_per_cu) stored for this grid shape; (0, 0): none."""
+        """(items_per_slot, wg_per_cu, energies_per_lane) stored for this grid
+        shape; (0, 0, 0): none."""
         sh = abi.LaunchShape()
         self._check(load().ppm_get_launch_shape(self._h, byref(grid), n_stars, atmosphere, byref(sh)))
-        return sh.items_per_slot, sh.wg_per_cu
+        return sh.items_per_slot, sh.wg_per_cu, sh.energies_per_lane
 
     # -- serving
     def submit(self, star, grid):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PPM_ABI_VERSION 1
+#define PPM_ABI_VERSION 2
 #define PPM_MAX_SPOTS 4
 
 typedef enum ppm_status {
@@ -323,10 +323,15 @@ ppm_status ppm_eval_fit(ppm_ctx* ctx, const ppm_star* stars, int n, const ppm_gr
 /* Launch shape of the batched path.  A 0 field means the default: 10
  * work-queue items per resident workgroup (each spot's rings are cut into
  * chunks of similar cost so that the queue holds about that many items per
- * workgroup), and the occupancy limit of workgroups per CU. */
+ * workgroup), the occupancy limit of workgroups per CU, and the fewest of 2,
+ * 4 or 6 energies per lane (128, 256 or 384 energies per work item) that
+ * hold the grid in one energy chunk with each row set of the NSX evaluation
+ * within 57 slice rows (else the most that stay within them). */
 typedef struct ppm_launch_shape {
-  int items_per_slot; /* work-queue items per resident workgroup (0..4096) */
-  int wg_per_cu;      /* resident workgroups per CU (0..64, capped by occupancy) */
+  int items_per_slot;    /* work-queue items per resident workgroup (0..4096)     */
+  int wg_per_cu;         /* resident workgroups per CU (0..64, capped by occupancy) */
+  int energies_per_lane; /* 0 (default), 2, 4 or 6                                  */
+  int reserved;          /* 0                                                       */
 } ppm_launch_shape;
 
 typedef struct ppm_tune_result {
@@ -341,8 +346,9 @@ typedef struct ppm_tune_result {
  * launch geometry is fixed (one block per ring, gpu/devarshi24st_gpu.cu:463,
  * gpu/os1_gpu.cu:316).  Times the batched integral of `stars` on `grid` over
  * a short coordinate search of the launch shape (items per workgroup halved
- * and doubled, one more step the better way, then one workgroup per CU
- * fewer), `repeats` runs each (0 -> 2), and keeps the fastest for this grid
+ * and doubled, one more step the better way, one workgroup per CU fewer,
+ * and, beyond 128 energies, the other energies-per-lane choices), `repeats`
+ * runs each (0 -> 2), and keeps the fastest for this grid
  * shape (grid fields, energy count, atmosphere of stars[0], bit length of
  * n): later batched evaluations and served launches of that shape use it.
  * Blocking; the fold is not run. */

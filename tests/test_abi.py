@@ -52,7 +52,7 @@ def test_struct_layouts_match_c(tmp_path):
 
 def test_strerror_covers_every_status():
     L = lib.load()
-    assert L.ppm_abi_version() == 1
+    assert L.ppm_abi_version() == 2
     for s in range(0, 11):
         assert L.ppm_strerror(s) and L.ppm_strerror(s) != b"unknown status"
     assert L.ppm_strerror(99) == b"unknown status"
