@@ -185,3 +185,25 @@ def test_batched_empty_and_hidden_spots(ctx, orc):
     assert stats.n_active > 0
     f0, c0, s0 = ctx.eval([empty, empty], g)
     assert s0.n_active == 0 and np.all(f0 == 0) and np.all(c0 == 0)
+
+
+def test_256_energies_four_per_lane_matches_oracle(ctx, orc):
+    """A 256-energy grid over 0.2-12 keV runs 4 energies per lane by default
+    (two row sets of 128 energies within the table evaluation's slice rows):
+    against the oracle, and equal to 2 per lane within 1e-5."""
+    stars = S.sample_priors(4, seed=12)
+    en = list(S.geomspace(0.2, 12.0, 256))
+    g = abi.make_grid(32, 16, sampling=abi.SAMPLE_GL, n_gl=2, energies=en, method=abi.METHOD_BATCHED)
+    print("case: batched 256 energies, N 32, 4 stars", flush=True)
+    assert ctx.launch_shape(g, len(stars), abi.ATM_NSX) == (0, 0, 0)
+    flux, _, _ = ctx.eval(stars, g)
+    for k, st in enumerate(stars):
+        ref, _ = orc.profile(st, g, threads=16)
+        assert rel_err(flux[k], ref) <= TOL, (k, rel_err(flux[k], ref))
+    ctx.set_launch_shape(g, len(stars), abi.ATM_NSX, 0, 0, 2)
+    try:
+        f2, _, _ = ctx.eval(stars, g)
+    finally:
+        ctx.set_launch_shape(g, len(stars), abi.ATM_NSX, 0, 0, 0)
+    for k in range(len(stars)):
+        assert rel_err(f2[k], flux[k]) <= 1e-5
