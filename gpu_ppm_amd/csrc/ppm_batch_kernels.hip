@@ -34,11 +34,15 @@ constexpr int kMaxEPL = 6;
 constexpr int kBSets = 2;
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
-// pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096).
+// pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096), and bit 31
+// (kEvInside) set when both ends of the event's window piece lie in the
+// same sweep segment.
 struct EvRec {
   uint32_t pb;
   float sw;
 };
+constexpr uint32_t kEvInside = 1u << 31;
+__device__ inline int ev_position(uint32_t pb) { return (int)((pb >> 12) & 0x7ffffu); }
 
 // Per-wavefront LDS of the sweep: the evaluation scratch and a queue of
 // covered positions (at most 63 pending + 64 new).
@@ -83,7 +87,7 @@ __device__ inline int covered_count(const uint32_t* s_cov, int n_words, int lane
 // Number of (sorted) events before position p.  Wave-uniform.
 __device__ inline int events_before(const EvRec* s_ev, int n_ev, int p, int lane) {
   int c = 0;
-  for (int i = lane; i < n_ev; i += 64) c += (int)(s_ev[i].pb >> 12) < p;
+  for (int i = lane; i < n_ev; i += 64) c += ev_position(s_ev[i].pb) < p;
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
   return c;
@@ -157,6 +161,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   float4* s_C = reinterpret_cast<float4*>(smem + lay.keys + wave * kSweepBytes);
   int* s_q = reinterpret_cast<int*>(smem + lay.keys + wave * kSweepBytes + kWaveScratchBytes);
   __shared__ int s_item, s_nev;
+  __shared__ int s_cut[kBWP];  // first covered position of each sweep segment
 
   if (ATM == PPM_ATM_NSX)
     for (int k = tid; k < N.nm; k += kBThreads) s_mu[k] = N.mu[k];
@@ -305,13 +310,29 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           }
           __syncthreads();
           const int n_ev = __builtin_amdgcn_readfirstlane(s_nev);
-          // ---- rank sort of the events (keys are unique) into records
-          for (int i = tid; i < n_ev; i += kBThreads) {
-            const uint64_t ki = s_keys[i];
-            int rank = 0;
-            for (int j = 0; j < n_ev; ++j) rank += s_keys[j] < ki;
-            const ClassPt cp = G.cpts[C.first_pt + g0 + (int)((ki >> 12) & 0xfffff)];
-            s_ev[rank] = EvRec{(uint32_t)(ki >> 32) << 12 | (uint32_t)cp.bin, (float)((ki & 1u) ? cp.w : -cp.w)};
+          // ---- rank sort of the events (keys are unique) into records.  The
+          // keys 2p, 2p + 1 are the two ends of one window piece: a thread
+          // ranks both in one pass and keeps its first pair's ranks and
+          // positions for the segment flags below
+          int pr_r0 = -1, pr_r1 = -1, pr_p0 = 0, pr_p1 = 0;
+          for (int p = tid; 2 * p < n_ev; p += kBThreads) {
+            const uint64_t k0 = s_keys[2 * p], k1 = s_keys[2 * p + 1];
+            int r0 = 0, r1 = 0;
+            for (int j = 0; j < n_ev; ++j) {
+              const uint64_t kj = s_keys[j];
+              r0 += kj < k0;
+              r1 += kj < k1;
+            }
+            const ClassPt cp = G.cpts[C.first_pt + g0 + (int)((k0 >> 12) & 0xfffff)];
+            const int p0 = (int)(k0 >> 32), p1 = (int)(k1 >> 32);
+            s_ev[r0] = EvRec{(uint32_t)p0 << 12 | (uint32_t)cp.bin, (float)-cp.w};  // window start: -w
+            s_ev[r1] = EvRec{(uint32_t)p1 << 12 | (uint32_t)cp.bin, (float)cp.w};   // window end: +w
+            if (p == tid) {
+              pr_r0 = r0;
+              pr_r1 = r1;
+              pr_p0 = p0;
+              pr_p1 = p1;
+            }
           }
           // ---- keep the covered positions whose patch is seen (flux/I != 0):
           // the others add nothing, so the split, the queue and the set-up
@@ -341,6 +362,24 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             const int n_cov = covered_count(s_cov, n_words, lane);
             if (wp > 0) seg_lo = covered_position(s_cov, n_words, n_cov * wp / kBWP, n, lane);
             if (wp < kBWP - 1) seg_hi = covered_position(s_cov, n_words, n_cov * (wp + 1) / kBWP, n, lane);
+            // a window piece whose two ends lie in one segment gets the
+            // earlier segments' sum at both ends with opposite signs: its
+            // events are marked, and the offset pass below skips them
+            if (lane == 0) s_cut[wp] = seg_lo;
+            __syncthreads();
+            if (pr_r0 >= 0) {
+              int s0 = 0, s1 = 0;
+#pragma unroll
+              for (int w = 1; w < kBWP; ++w) {
+                s0 += pr_p0 >= s_cut[w];
+                s1 += pr_p1 >= s_cut[w];
+              }
+              if (s0 == s1) {
+                s_ev[pr_r0].pb |= kEvInside;
+                s_ev[pr_r1].pb |= kEvInside;
+              }
+            }
+            __syncthreads();
           }
           const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
           const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
@@ -355,6 +394,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           uint32_t v_pb = 0;
           float v_sw = 0.f;
           int ev_pos = 0x7fffffff, ev_bin = 0;
+          bool ev_inside = false;
           double ev_sw = 0.0;
           auto load_ev = [&]() {
             if (ev < ev_end) {
@@ -367,8 +407,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
                 }
               }
               const uint32_t pb = (uint32_t)readlane_i((int)v_pb, ev - win);
-              ev_pos = (int)(pb >> 12);
+              ev_pos = ev_position(pb);
               ev_bin = (int)(pb & 4095u);
+              ev_inside = (pb & kEvInside) != 0;
               ev_sw = (double)readlane_f(v_sw, ev - win);
             } else {
               ev_pos = 0x7fffffff;
@@ -523,7 +564,14 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
               ev = ev_begin;
               win = ev_begin - 64;
               load_ev();
-              while (ev < ev_end) fire();
+              while (ev < ev_end) {
+                if (ev_inside) {  // the window's other end is in this segment too
+                  ++ev;
+                  load_ev();
+                } else {
+                  fire();
+                }
+              }
             }
             __syncthreads();  // the scratch is read before the next group writes it
           }

@@ -142,15 +142,16 @@ def test_budget_admission(data_root, small_nsx_cfg):
 
 
 def test_light_request_overtakes_a_heavy_launch(ctx, data_root, small_nsx_cfg):
-    """A light request submitted while a heavy N_side 2048 request runs on
+    """A light request submitted while a heavy N_side 4096 request runs on
     the device starts in the workgroup slots the heavy launch leaves free and
     finishes first; both results equal their direct evaluations and the
-    light one the oracle."""
-    heavy_g = abi.make_grid(2048, 128, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
+    light one the oracle.  (The heavy request takes about 0.1 s; at N_side
+    2048 it ends too close to the light one for the order to be a test.)"""
+    heavy_g = abi.make_grid(4096, 128, sampling=abi.SAMPLE_GL, n_gl=2, response=0)
     light_g = abi.make_grid(64, 16, sampling=abi.SAMPLE_GL, n_gl=2, response=1)
     base = S.sample_priors(3, seed=51)
     heavy, warm, light = base
-    heavy.spot[0].angrad1 = 1.0   # about 1e12 evaluations: far above the light-work threshold
+    heavy.spot[0].angrad1 = 1.0   # about 4e12 evaluations: far above the light-work threshold
     warm.spot[0].angrad1 = 0.6
     light.spot[0].angrad1 = 0.3
     print("case: warm-up of both grids", flush=True)
