@@ -294,6 +294,26 @@ static ppm_status class_tables(ppm_ctx* c, BatchState* BS, const ppm_grid& g, co
 
 constexpr int kEvCap = 512;  // events per class group in LDS
 
+// Class boundaries of k_integral_batched's sweep groups on a ring of npix
+// pixels (its rule for rings of at most kGroupRuns runs: whole classes,
+// starting from class 0, while their slots fit kGroupPositions and their
+// points kEvCap / (4 kGroupRuns)): bounds = 0, ..., the ring's class count.
+// A ring split into items by class is cut only there, so that its groups,
+// and with them the order of every sum, are the ones of the whole ring.
+static void class_groups(const ClassTables& T, int npix, std::vector<int>& bounds) {
+  const ClassSet cs = T.sets[npix / 4];
+  const int n_pad = (npix + 63) & ~63;
+  const int slot_cap = std::max(1, std::min(kMaxSlots, kGroupPositions / n_pad));
+  const int grp_pts = kEvCap / (4 * kGroupRuns);
+  bounds.assign(1, 0);
+  for (int ci = 0; ci < cs.n_classes;) {
+    int cj = ci + 1, pts = T.hdr[cs.first_class + ci].n_pts;
+    while (cj < cs.n_classes && cj - ci < slot_cap && pts + T.hdr[cs.first_class + cj].n_pts <= grp_pts)
+      pts += T.hdr[cs.first_class + cj++].n_pts;
+    bounds.push_back(ci = cj);
+  }
+}
+
 // True when, at epl energies per lane, every row set of a work item (epl/2 x
 // 64 consecutive energies of a 64 epl-energy chunk) spans at most the slice
 // rows the table evaluation takes (58: WaveEnergies::fast; 57 here, below
@@ -340,7 +360,7 @@ static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& g
   const double h = (c->N.logE_max - c->N.logE_min) / std::max(c->N.nE - 1, 1);
   for (const BatchGroup& G : groups) {
     nf_max = std::max(nf_max, G.g.n_fine);
-    cov_words = std::max(cov_words, (4 * G.g.n_side + 63) / 64 * 2 + 2);
+    cov_words = std::max(cov_words, std::max((4 * G.g.n_side + 63) / 64 * 2, kGroupPositions / 32) + 2);
     if (atm == PPM_ATM_NSX && !rows_override) {
       double span = 0;
       const int ne = (int)G.energies.size();
@@ -355,7 +375,7 @@ static size_t launch_geometry(const ppm_ctx* c, const std::vector<BatchGroup>& g
     if (rows_override) nsx_rows = rows_override;
     nsx_rows = std::max(nsx_rows, 64);  // a row set's 64 lanes read inside the allocation
   }
-  return batch_lds_bytes(nf_max, atm, nsx_rows, cov_words, kEvCap, fine_lds);
+  return batch_lds_bytes(nf_max, atm, c->N.nm, nsx_rows, cov_words, kEvCap, fine_lds);
 }
 
 int batch_grid_epl(const ppm_ctx* c, const BatchGroup& G) {
@@ -514,6 +534,7 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   std::vector<int> star_item0(n + 1, 0);
   size_t acc_total = 0;
   std::vector<double> cuts;
+  std::vector<int> bounds;
   // one item per energy chunk of tasks [t0, t1), classes [c0, c1) of each ring
   auto add_item = [&](int t0, int t1, int s, int si, int c0, int c1, double ic) {
     const int k_grid = spot_group[si];
@@ -549,9 +570,26 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
             int m = 1;
             if (n_cls[t] > 1 && cost[t] > 1.5 * unit)
               m = (int)std::min<double>(n_cls[t], std::max(1.0, std::round(cost[t] / unit)));
-            for (int j = 0; j < m; ++j)
-              add_item(t, t + 1, s, si, m > 1 ? j * n_cls[t] / m : 0, m > 1 ? (j + 1) * n_cls[t] / m : kAllClasses,
-                       cost[t] / m);
+            if (m == 1) {
+              add_item(t, t + 1, s, si, 0, kAllClasses, cost[t]);
+              continue;
+            }
+            // m pieces cut at the sweep-group boundaries nearest to j n / m
+            const int r = tasks[t].ring, ns = groups[k_grid].g.n_side, npix = 4 * std::min({r, 4 * ns - r, ns});
+            class_groups(cls[k_grid]->tab, npix, bounds);
+            int c0 = 0;
+            for (int j = 1; j <= m; ++j) {
+              const double target = (double)j * n_cls[t] / m;
+              int c1 = n_cls[t];
+              if (j < m) {
+                const auto it = std::lower_bound(bounds.begin(), bounds.end(), (int)std::lround(target));
+                c1 = it == bounds.end() ? n_cls[t] : *it;
+                if (it != bounds.begin() && std::fabs(*(it - 1) - target) < std::fabs(c1 - target)) c1 = *(it - 1);
+              }
+              if (c1 <= c0) continue;
+              add_item(t, t + 1, s, si, c0, c1, cost[t] * (c1 - c0) / n_cls[t]);
+              c0 = c1;
+            }
           }
           continue;
         }

@@ -9,6 +9,19 @@
 
 namespace ppm {
 
+// The batched integral sweeps several phase classes of a ring together, each
+// in a slot of its own on a virtual position axis: at most kMaxSlots classes
+// whose slots span at most kGroupPositions positions (a class of a ring with
+// more pixels is swept alone), with at most ev_cap / (4 kGroupRuns) phase
+// points, and only on rings of at most kGroupRuns runs of active pixels
+// (others sweep one class at a time).  The groups of a ring follow from its
+// pixel count and class table alone (class_groups on the host gives the same
+// boundaries), so that a ring split into items by class is cut between
+// groups.
+constexpr int kGroupPositions = 8192;
+constexpr int kMaxSlots = 32;
+constexpr int kGroupRuns = 2;
+
 // Phase points of a grid grouped by their fractional offset on a ring of n
 // pixels: p n = m + delta.  One ClassSet per pixel count n (index 0: the
 // one-element point source, index t: n = 4t).
@@ -74,7 +87,7 @@ struct BatchLaunch {
   unsigned long long* rows_flag;  // bit 1 set when a ring needs more NSX rows than staged
 };
 
-size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap, bool fine_lds);
+size_t batch_lds_bytes(int n_fine, int atmosphere, int nm, int nsx_rows, int cov_words, int ev_cap, bool fine_lds);
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals

@@ -15,6 +15,7 @@
 // only reassociates the reference's sum (cpu/devarshi24st_cpu.cpp:214-239).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "ppm_batch_dev.hpp"
@@ -94,16 +95,25 @@ __device__ inline int events_before(const EvRec* s_ev, int n_ev, int p, int lane
 }
 
 struct BLds {
-  size_t ph, mu, fI, zf, cs, S, cov, keys, sorted, total;
+  size_t ph, mu, hidx, grp, fI, zf, cs, S, cov, keys, sorted, total;
 };
 
-__host__ __device__ inline BLds batch_layout(int nf1, int nsx_rows, int cov_words, int ev_cap) {
+// LDS of the nm mu nodes and their [nm][4] inverse Lagrange denominators
+// (the layout of DevNsx::mu), and the bucket indexes of the mu nodes and of
+// the ring's phase table.
+
+__host__ __device__ inline BLds batch_layout(int nf1, int nm, int nsx_rows, int cov_words, int ev_cap) {
   BLds l;
   size_t o = 0;
   l.ph = o;
   o += (size_t)nf1 * 8;
   l.mu = o;
-  o += 80 * 8;
+  o += (size_t)nm * 5 * 8;
+  l.hidx = o;
+  o += 2 * (kHuntBuckets + 1) * sizeof(uint16_t);
+  o = (o + 7) & ~(size_t)7;
+  l.grp = o;  // phase offset of each class slot, then the slot of each point of a group
+  o += kMaxSlots * 8 + (size_t)ev_cap / 4;
   l.fI = o;
   o += (size_t)nf1 * 4;
   l.zf = o;
@@ -148,9 +158,14 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   const int wp = wave;  // position part
   extern __shared__ __align__(16) unsigned char smem[];
   const int rows_alloc = ATM == PPM_ATM_NSX ? B.nsx_rows : 0;
-  const BLds lay = batch_layout(FL ? B.n_fine_max + 1 : 0, rows_alloc, B.cov_words, B.ev_cap);
+  const BLds lay = batch_layout(FL ? B.n_fine_max + 1 : 0, ATM == PPM_ATM_NSX ? N.nm : 0, rows_alloc, B.cov_words,
+                                B.ev_cap);
   double* s_ph = reinterpret_cast<double*>(smem + lay.ph);
   double* s_mu = reinterpret_cast<double*>(smem + lay.mu);
+  uint16_t* s_mu_idx = reinterpret_cast<uint16_t*>(smem + lay.hidx);
+  uint16_t* s_ph_idx = s_mu_idx + kHuntBuckets + 1;
+  double* s_co = reinterpret_cast<double*>(smem + lay.grp);
+  uint8_t* s_pslot = reinterpret_cast<uint8_t*>(smem + lay.grp + kMaxSlots * 8);
   float* s_fI = reinterpret_cast<float*>(smem + lay.fI);
   float* s_zf = reinterpret_cast<float*>(smem + lay.zf);
   float* s_cs = reinterpret_cast<float*>(smem + lay.cs);
@@ -163,8 +178,12 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   __shared__ int s_item, s_nev;
   __shared__ int s_cut[kBWP];  // first covered position of each sweep segment
 
-  if (ATM == PPM_ATM_NSX)
-    for (int k = tid; k < N.nm; k += kBThreads) s_mu[k] = N.mu[k];
+  HuntIndex MU{nullptr, 0.0, 0.0};
+  if (ATM == PPM_ATM_NSX) {
+    for (int k = tid; k < 5 * N.nm; k += kBThreads) s_mu[k] = N.mu[k];  // nodes, then denominators
+    __syncthreads();
+    MU = build_hunt_index(s_mu, N.nm, s_mu_idx, tid, kBThreads);  // read after the item loop's barrier
+  }
 
   const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
   const double inv_h = 1.0 / h;
@@ -246,6 +265,8 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const int mu_mode = nsx_mu_mode(S.nsx_interp);
       const RingTab R = FL ? RingTab{s_ph, s_fI, s_zf, s_cs, nf1} : RingTab{gph, gf, gf + nf1, gf + 2 * nf1, nf1};
       const double ph0 = R.ph[0];
+      // read after the first class group's barrier
+      const HuntIndex PH = build_hunt_index(R.ph, nf1, s_ph_idx, tid, kBThreads);
       const bool point = T.ring == 0;
       const bool ring_hidden = Dp->hidden != 0;
       const int n = point ? 1 : Dp->n_pix;
@@ -253,328 +274,373 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       const int n_runs = point ? 1 : Dp->n_runs;
       const ClassSet cset = G.csets[point ? 0 : n / 4];
       const double c_shift = point ? Dp->ps_shift : 1.0 - 0.25 * Dp->s_shift2;
-      const int n_words = (n + 63) / 64 * 2;  // coverage words, 64-position batches
 
+      // Classes of the ring are swept together, each in a slot of a virtual
+      // position axis (slot s holds positions s n_pad + k, k < n; n_pad is n
+      // rounded up to 64, so a 64-position chunk lies in one slot).  A window
+      // never leaves its slot, so one running prefix serves every class of a
+      // group.  A group holds whole classes while their slots fit
+      // kGroupPositions and their points grp_pts (rings of at most
+      // kGroupRuns runs), or one class, or the points [g0, g1) of one class
+      // too large for the event list (pt_cap).  The grouping depends on the
+      // ring and the class table only.
+      const int n_pad = (n + 63) & ~63;
+      const int slot_cap = max(1, min(kMaxSlots, kGroupPositions / n_pad));
+      const int pt_cap = max(1, B.ev_cap / (4 * n_runs));
+      const int grp_pts = n_runs <= kGroupRuns ? B.ev_cap / (4 * kGroupRuns) : 0;
+      const float inv_npad = 1.f / (float)n_pad;
       const int c_end = min(It.c1, cset.n_classes);
-      for (int ci = It.c0; ci < c_end; ++ci) {
+      for (int ci = It.c0, g0 = 0; ci < c_end;) {
         const ClassHdr C = G.chdr[cset.first_class + ci];
-        const double c_o = C.delta + c_shift;
-        // rotational phase of position k, wrapped into [ph[0], ph[0] + 1)
-        auto phase_of = [&](int k) {
-          const double v = (k + c_o) * inv_n + 1.0;
+        int cj = ci + 1, g1 = C.n_pts, n_pts = C.n_pts - g0;
+        if (n_pts > pt_cap) {
+          g1 = g0 + pt_cap;
+          n_pts = pt_cap;
+        } else if (g0 == 0) {  // (the tail of a class swept in chunks stays alone)
+          while (cj < c_end && cj - ci < slot_cap) {
+            const int more = G.chdr[cset.first_class + cj].n_pts;
+            if (n_pts + more > grp_pts) break;
+            n_pts += more;
+            ++cj;
+          }
+        }
+        const int n_slots = cj - ci;
+        const int pt0 = C.first_pt + g0;  // the classes' points are consecutive in the table
+        const int nv = n_slots * n_pad;
+        const int n_words = nv / 32;  // coverage words (64-position chunks)
+        const int n_win = n_pts * n_runs;
+        // rotational phase of position k of slot s, wrapped into [ph[0], ph[0] + 1)
+        auto phase_of = [&](int k, int s) {
+          const double v = (k + s_co[s]) * inv_n + 1.0;
           double pp = v - floor(v);
           if (pp < ph0) pp += 1.;
           return pp;
         };
-        const int per_group = max(1, B.ev_cap / (4 * n_runs));
-        for (int g0 = 0; g0 < C.n_pts; g0 += per_group) {
-          const int g1 = min(C.n_pts, g0 + per_group);
-          const int n_win = (g1 - g0) * n_runs;
-          // ---- windows -> events and coverage
-          __syncthreads();
-          for (int w = tid; w < n_words; w += kBThreads) s_cov[w] = 0u;
-          if (tid == 0) s_nev = 0;
-          __syncthreads();
-          for (int wi = tid; wi < n_win; wi += kBThreads) {
-            const int ql = wi / n_runs, r = wi - ql * n_runs;
-            const ClassPt cp = G.cpts[C.first_pt + g0 + ql];
-            const int j0 = point ? 1 : Dp->run_start[r];
-            const int L = point ? 1 : Dp->run_len[r];
-            int a = cp.m + j0 - 1;
-            a %= n;
-            const uint64_t qr = ((uint64_t)ql << 12) | ((uint64_t)r << 2);
-            int pieces[2][2];
-            int np = 0;
-            if (a + L <= n) {
-              pieces[np][0] = a;
-              pieces[np++][1] = a + L;
-            } else {
-              pieces[np][0] = a;
-              pieces[np++][1] = n;
-              pieces[np][0] = 0;
-              pieces[np++][1] = a + L - n;
-            }
-            for (int pc = 0; pc < np; ++pc) {
-              const int lo = pieces[pc][0], hi = pieces[pc][1];
-              const int slot = atomicAdd(&s_nev, 2);
-              s_keys[slot] = ((uint64_t)lo << 32) | qr | ((uint64_t)pc << 1) | 0u;
-              s_keys[slot + 1] = ((uint64_t)hi << 32) | qr | ((uint64_t)pc << 1) | 1u;
-              for (int k = lo; k < hi;) {  // coverage bits
-                const int wd = k >> 5, b0 = k & 31;
-                const int nb = min(32 - b0, hi - k);
-                const uint32_t m = (nb == 32 ? 0xffffffffu : ((1u << nb) - 1u)) << b0;
-                atomicOr(&s_cov[wd], m);
-                k += nb;
-              }
+        // ---- windows -> events and coverage
+        __syncthreads();
+        for (int w = tid; w < n_words; w += kBThreads) s_cov[w] = 0u;
+        if (tid == 0) s_nev = 0;
+        for (int s = 0, off = 0; s < n_slots; ++s) {  // slot of each point, phase offset of each slot
+          const ClassHdr Cs = G.chdr[cset.first_class + ci + s];
+          const int cnt = s == 0 ? g1 - g0 : Cs.n_pts;
+          for (int q = tid; q < cnt; q += kBThreads) s_pslot[off + q] = (uint8_t)s;
+          if (tid == 0) s_co[s] = Cs.delta + c_shift;
+          off += cnt;
+        }
+        __syncthreads();
+        for (int wi = tid; wi < n_win; wi += kBThreads) {
+          const int ql = wi / n_runs, r = wi - ql * n_runs;
+          const ClassPt cp = G.cpts[pt0 + ql];
+          const int vb = s_pslot[ql] * n_pad;  // the slot's first virtual position
+          const int j0 = point ? 1 : Dp->run_start[r];
+          const int L = point ? 1 : Dp->run_len[r];
+          int a = cp.m + j0 - 1;
+          a %= n;
+          const uint64_t qr = ((uint64_t)ql << 12) | ((uint64_t)r << 2);
+          int pieces[2][2];
+          int np = 0;
+          if (a + L <= n) {
+            pieces[np][0] = a;
+            pieces[np++][1] = a + L;
+          } else {
+            pieces[np][0] = a;
+            pieces[np++][1] = n;
+            pieces[np][0] = 0;
+            pieces[np++][1] = a + L - n;
+          }
+          for (int pc = 0; pc < np; ++pc) {
+            const int lo = vb + pieces[pc][0], hi = vb + pieces[pc][1];
+            const int slot = atomicAdd(&s_nev, 2);
+            s_keys[slot] = ((uint64_t)lo << 32) | qr | ((uint64_t)pc << 1) | 0u;
+            s_keys[slot + 1] = ((uint64_t)hi << 32) | qr | ((uint64_t)pc << 1) | 1u;
+            for (int k = lo; k < hi;) {  // coverage bits
+              const int wd = k >> 5, b0 = k & 31;
+              const int nb = min(32 - b0, hi - k);
+              const uint32_t m = (nb == 32 ? 0xffffffffu : ((1u << nb) - 1u)) << b0;
+              atomicOr(&s_cov[wd], m);
+              k += nb;
             }
           }
-          __syncthreads();
-          const int n_ev = __builtin_amdgcn_readfirstlane(s_nev);
-          // ---- rank sort of the events (keys are unique) into records.  The
-          // keys 2p, 2p + 1 are the two ends of one window piece: a thread
-          // ranks both in one pass and keeps its first pair's ranks and
-          // positions for the segment flags below
-          int pr_r0 = -1, pr_r1 = -1, pr_p0 = 0, pr_p1 = 0;
-          for (int p = tid; 2 * p < n_ev; p += kBThreads) {
-            const uint64_t k0 = s_keys[2 * p], k1 = s_keys[2 * p + 1];
-            int r0 = 0, r1 = 0;
-            for (int j = 0; j < n_ev; ++j) {
-              const uint64_t kj = s_keys[j];
-              r0 += kj < k0;
-              r1 += kj < k1;
-            }
-            const ClassPt cp = G.cpts[C.first_pt + g0 + (int)((k0 >> 12) & 0xfffff)];
-            const int p0 = (int)(k0 >> 32), p1 = (int)(k1 >> 32);
-            s_ev[r0] = EvRec{(uint32_t)p0 << 12 | (uint32_t)cp.bin, (float)-cp.w};  // window start: -w
-            s_ev[r1] = EvRec{(uint32_t)p1 << 12 | (uint32_t)cp.bin, (float)cp.w};   // window end: +w
-            if (p == tid) {
-              pr_r0 = r0;
-              pr_r1 = r1;
-              pr_p0 = p0;
-              pr_p1 = p1;
-            }
+        }
+        __syncthreads();
+        const int n_ev = __builtin_amdgcn_readfirstlane(s_nev);
+        // ---- rank sort of the events (keys are unique) into records.  The
+        // keys 2p, 2p + 1 are the two ends of one window piece: a thread
+        // ranks both in one pass and keeps its first pair's ranks and
+        // positions for the segment flags below
+        int pr_r0 = -1, pr_r1 = -1, pr_p0 = 0, pr_p1 = 0;
+        for (int p = tid; 2 * p < n_ev; p += kBThreads) {
+          const uint64_t k0 = s_keys[2 * p], k1 = s_keys[2 * p + 1];
+          int r0 = 0, r1 = 0;
+          for (int j = 0; j < n_ev; ++j) {
+            const uint64_t kj = s_keys[j];
+            r0 += kj < k0;
+            r1 += kj < k1;
           }
-          // ---- keep the covered positions whose patch is seen (flux/I != 0):
-          // the others add nothing, so the split, the queue and the set-up
-          // only count patches that contribute.  Each wavefront rewrites its
-          // own 64-position chunks.  A ring without invisible fine nodes has
-          // flux/I > 0 between all of them: nothing to drop.
-          for (int base = wave * 64; ring_hidden && base < n; base += kBThreads) {
-            const int k = base + lane;
-            bool keep = false;
-            if (k < n && ((s_cov[k >> 5] >> (k & 31)) & 1u)) {
-              int fi;
-              double fa;
-              keep = patch_flux_factor(phase_of(k), R, fi, fa) != 0;
-            }
-            const uint64_t m = __ballot(keep);
-            if (lane == 0) s_cov[base >> 5] = (uint32_t)m;
-            if (lane == 32) s_cov[(base >> 5) + 1] = (uint32_t)(m >> 32);
+          const ClassPt cp = G.cpts[pt0 + (int)((k0 >> 12) & 0xfffff)];
+          const int p0 = (int)(k0 >> 32), p1 = (int)(k1 >> 32);
+          s_ev[r0] = EvRec{(uint32_t)p0 << 12 | (uint32_t)cp.bin, (float)-cp.w};  // window start: -w
+          s_ev[r1] = EvRec{(uint32_t)p1 << 12 | (uint32_t)cp.bin, (float)cp.w};   // window end: +w
+          if (p == tid) {
+            pr_r0 = r0;
+            pr_r1 = r1;
+            pr_p0 = p0;
+            pr_p1 = p1;
           }
-          __syncthreads();
+        }
+        // ---- keep the covered positions whose patch is seen (flux/I != 0):
+        // the others add nothing, so the split, the queue and the set-up
+        // only count patches that contribute.  Each wavefront rewrites its
+        // own 64-position chunks.  A ring without invisible fine nodes has
+        // flux/I > 0 between all of them: nothing to drop.
+        for (int base = wave * 64; ring_hidden && base < nv; base += kBThreads) {
+          const int v = base + lane, s = base / n_pad, k = v - s * n_pad;
+          bool keep = false;
+          if (k < n && ((s_cov[v >> 5] >> (v & 31)) & 1u)) {
+            int fi;
+            double fa;
+            keep = patch_flux_factor(phase_of(k, s), R, fi, fa, PH) != 0;
+          }
+          const uint64_t m = __ballot(keep);
+          if (lane == 0) s_cov[base >> 5] = (uint32_t)m;
+          if (lane == 32) s_cov[(base >> 5) + 1] = (uint32_t)(m >> 32);
+        }
+        __syncthreads();
 
-          // ---- sweep.  The covered positions are split in kBWP segments of
-          // equal count, [seg_lo, seg_hi) for this wavefront; it takes the
-          // events in its segment with a prefix that starts at zero, and the
-          // sum of the earlier segments is added to those events afterwards.
-          int seg_lo = 0, seg_hi = 0x7fffffff;
-          if (kBWP > 1) {
-            const int n_cov = covered_count(s_cov, n_words, lane);
-            if (wp > 0) seg_lo = covered_position(s_cov, n_words, n_cov * wp / kBWP, n, lane);
-            if (wp < kBWP - 1) seg_hi = covered_position(s_cov, n_words, n_cov * (wp + 1) / kBWP, n, lane);
-            // a window piece whose two ends lie in one segment gets the
-            // earlier segments' sum at both ends with opposite signs: its
-            // events are marked, and the offset pass below skips them
-            if (lane == 0) s_cut[wp] = seg_lo;
-            __syncthreads();
-            if (pr_r0 >= 0) {
-              int s0 = 0, s1 = 0;
+        // ---- sweep.  The covered positions are split in kBWP segments of
+        // equal count, [seg_lo, seg_hi) for this wavefront; it takes the
+        // events in its segment with a prefix that starts at zero, and the
+        // sum of the earlier segments is added to those events afterwards.
+        int seg_lo = 0, seg_hi = 0x7fffffff;
+        if (kBWP > 1) {
+          const int n_cov = covered_count(s_cov, n_words, lane);
+          if (wp > 0) seg_lo = covered_position(s_cov, n_words, n_cov * wp / kBWP, nv, lane);
+          if (wp < kBWP - 1) seg_hi = covered_position(s_cov, n_words, n_cov * (wp + 1) / kBWP, nv, lane);
+          // a window piece whose two ends lie in one segment gets the
+          // earlier segments' sum at both ends with opposite signs: its
+          // events are marked, and the offset pass below skips them
+          if (lane == 0) s_cut[wp] = seg_lo;
+          __syncthreads();
+          if (pr_r0 >= 0) {
+            int s0 = 0, s1 = 0;
 #pragma unroll
-              for (int w = 1; w < kBWP; ++w) {
-                s0 += pr_p0 >= s_cut[w];
-                s1 += pr_p1 >= s_cut[w];
-              }
-              if (s0 == s1) {
-                s_ev[pr_r0].pb |= kEvInside;
-                s_ev[pr_r1].pb |= kEvInside;
-              }
+            for (int w = 1; w < kBWP; ++w) {
+              s0 += pr_p0 >= s_cut[w];
+              s1 += pr_p1 >= s_cut[w];
             }
-            __syncthreads();
+            if (s0 == s1) {
+              s_ev[pr_r0].pb |= kEvInside;
+              s_ev[pr_r1].pb |= kEvInside;
+            }
           }
-          const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
-          const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
-          double P[kBEPL];
+          __syncthreads();
+        }
+        const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
+        const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
+        double P[kBEPL];
 #pragma unroll
-          for (int u = 0; u < kBEPL; ++u) P[u] = 0.0;
-          // 64 events at a time are held one per lane; the next one is read
-          // into scalar registers.  Contributions are posted as no-return
-          // FP64 atomics to the item's private accumulator (one writer per
-          // address: deterministic)
-          int ev = ev_begin, win = ev_begin - 64;
-          uint32_t v_pb = 0;
-          float v_sw = 0.f;
-          int ev_pos = 0x7fffffff, ev_bin = 0;
-          bool ev_inside = false;
-          double ev_sw = 0.0;
-          auto load_ev = [&]() {
-            if (ev < ev_end) {
-              if (ev - win >= 64) {
-                win = ev;
-                if (ev + lane < n_ev) {
-                  const EvRec r = s_ev[ev + lane];
-                  v_pb = r.pb;
-                  v_sw = r.sw;
-                }
+        for (int u = 0; u < kBEPL; ++u) P[u] = 0.0;
+        // 64 events at a time are held one per lane; the next one is read
+        // into scalar registers.  Contributions are posted as no-return
+        // FP64 atomics to the item's private accumulator (one writer per
+        // address: deterministic)
+        int ev = ev_begin, win = ev_begin - 64;
+        uint32_t v_pb = 0;
+        float v_sw = 0.f;
+        int ev_pos = 0x7fffffff, ev_bin = 0;
+        bool ev_inside = false;
+        double ev_sw = 0.0;
+        auto load_ev = [&]() {
+          if (ev < ev_end) {
+            if (ev - win >= 64) {
+              win = ev;
+              if (ev + lane < n_ev) {
+                const EvRec r = s_ev[ev + lane];
+                v_pb = r.pb;
+                v_sw = r.sw;
               }
-              const uint32_t pb = (uint32_t)readlane_i((int)v_pb, ev - win);
-              ev_pos = ev_position(pb);
-              ev_bin = (int)(pb & 4095u);
-              ev_inside = (pb & kEvInside) != 0;
-              ev_sw = (double)readlane_f(v_sw, ev - win);
-            } else {
-              ev_pos = 0x7fffffff;
             }
-          };
-          auto fire = [&]() {
-            double* const row = acc + lane + ev_bin * en;  // the energies' offsets are immediates
+            const uint32_t pb = (uint32_t)readlane_i((int)v_pb, ev - win);
+            ev_pos = ev_position(pb);
+            ev_bin = (int)(pb & 4095u);
+            ev_inside = (pb & kEvInside) != 0;
+            ev_sw = (double)readlane_f(v_sw, ev - win);
+          } else {
+            ev_pos = 0x7fffffff;
+          }
+        };
+        auto fire = [&]() {
+          double* const row = acc + lane + ev_bin * en;  // the energies' offsets are immediates
 #pragma unroll
-            for (int u = 0; u < kBEPL; ++u)
-              if (u * 64 + lane <= e_lim) unsafeAtomicAdd(row + u * 64, ev_sw * P[u]);
-            ++ev;
-            load_ev();
-          };
+          for (int u = 0; u < kBEPL; ++u)
+            if (u * 64 + lane <= e_lim) unsafeAtomicAdd(row + u * 64, ev_sw * P[u]);
+          ++ev;
           load_ev();
-          // covered positions, in ascending order, 64 per batch: one patch
-          // set-up per lane, then every lane evaluates each visible patch
-          auto run_batch = [&](int cnt, int q_len) {
-            const int k = lane < cnt ? s_q[lane] : 0;
-            PatchVals pv;
-            pv.im = -1;
-            if (lane < cnt) patch_setup<ATM>(phase_of(k), 1.0, R, s_mu, N.mu_den, N.nm, N.mu_min, mu_mode, txm, pv);
-            // skipped patches add nothing to the prefix: the events between
-            // two queued positions fire with the same prefix
-            uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
-            n_done += n_valid * (unsigned int)__builtin_popcountll(todo);  // every visible patch is evaluated
-            uint64_t due = __ballot(k >= ev_pos);
-            PatchRows<kBSets> pr{};
-            if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
-            // patches evaluated without clamps, and those whose low row set
-            // takes the clamped evaluation (the rest clamp the high set only:
-            // energies beyond the end of the axis)
-            static_assert(kBSets == 2, "the clamp classes assume two row sets");
-            const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.clamp == 0);
-            const uint64_t clamp_lo = __ballot(lane < cnt && pv.im >= 0 && (pr.clamp & 1));
-            // FP32 partial sums of at most one batch, folded into the FP64
-            // prefix before an event fires and at the end of the batch
-            float pf[kBEPL];
+        };
+        load_ev();
+        // covered positions, in ascending order, 64 per batch: one patch
+        // set-up per lane, then every lane evaluates each visible patch
+        auto run_batch = [&](int cnt, int q_len) {
+          const int k = lane < cnt ? s_q[lane] : 0;  // virtual position
+          PatchVals pv;
+          pv.im = -1;
+          if (lane < cnt) {
+            const int s = n_slots == 1 ? 0 : (int)(((float)k + 0.5f) * inv_npad);
+            patch_setup<ATM>(phase_of(k - s * n_pad, s), 1.0, R, s_mu, s_mu + N.nm, N.nm, N.mu_min, mu_mode, txm, pv,
+                             PH, MU);
+          }
+          // skipped patches add nothing to the prefix: the events between
+          // two queued positions fire with the same prefix
+          uint64_t todo = __ballot(lane < cnt && pv.im >= 0);
+          n_done += n_valid * (unsigned int)__builtin_popcountll(todo);  // every visible patch is evaluated
+          uint64_t due = __ballot(k >= ev_pos);
+          PatchRows<kBSets> pr{};
+          if (ATM == PPM_ATM_NSX && pv.im >= 0) pr = patch_rows(pv.tx, pv.im, we, ax);
+          // patches evaluated without clamps, and those whose low row set
+          // takes the clamped evaluation (the rest clamp the high set only:
+          // energies beyond the end of the axis)
+          static_assert(kBSets == 2, "the clamp classes assume two row sets");
+          const uint64_t interior = __ballot(lane < cnt && pv.im >= 0 && pr.clamp == 0);
+          const uint64_t clamp_lo = __ballot(lane < cnt && pv.im >= 0 && (pr.clamp & 1));
+          // FP32 partial sums of at most one batch, folded into the FP64
+          // prefix before an event fires and at the end of the batch
+          float pf[kBEPL];
 #pragma unroll
-            for (int u = 0; u < kBEPL; ++u) pf[u] = 0.f;
-            auto fold = [&]() {
+          for (int u = 0; u < kBEPL; ++u) pf[u] = 0.f;
+          auto fold = [&]() {
 #pragma unroll
-              for (int u = 0; u < kBEPL; ++u) {
-                P[u] += (double)pf[u];
-                pf[u] = 0.f;
-              }
-            };
-            // in position order: window ends at or before pos use the
-            // prefix before F(pos); `due` marks the lanes whose position is
-            // at or after the next window end
-            auto accumulate = [&](int q, const float (&val)[kBEPL]) {
-              if (__builtin_expect((due >> q) & 1, 0)) {
-                const int pos = readlane_i(k, q);
-                fold();
-                while (ev_pos <= pos) fire();
-                due = __ballot(k >= ev_pos);
-              }
-#pragma unroll
-              for (int u = 0; u < kBEPL; ++u) pf[u] += val[u];
-            };
-            // one patch at a time.  With the row-set path the slice rows of
-            // the next patch are read while the current one is evaluated (two
-            // row buffers, used in turn); the path is chosen once per batch
-            if (ATM == PPM_ATM_NSX && we.fast) {
-              auto step_rows = [&](const float (&p_cur)[kBSets][4], float (&p_next)[kBSets][4]) {
-                const int q = __builtin_ctzll(todo);
-                todo &= todo - 1;
-                if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
-                float val[kBEPL];
-                if (__builtin_expect((interior >> q) & 1, 1))
-                  nsx_eval_table<kBEPL, 0, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
-                else if ((clamp_lo >> q) & 1)
-                  nsx_eval_table<kBEPL, 3, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
-                else
-                  nsx_eval_table<kBEPL, 2, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
-                accumulate(q, val);
-              };
-              float p_a[kBSets][4], p_b[kBSets][4];
-              if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a);
-              while (todo) {
-                step_rows(p_a, p_b);
-                if (!todo) break;
-                step_rows(p_b, p_a);
-              }
-            } else {
-              while (todo) {
-                const int q = __builtin_ctzll(todo);
-                todo &= todo - 1;
-                float val[kBEPL];
-                if (ATM == PPM_ATM_NSX) {
-                  float w[4];
-#pragma unroll
-                  for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
-                  nsx_eval_edge<kBEPL, EM>(le, asc, readlane_f(pv.tx, q), readlane_i(pv.im, q), w, ax, s_S, s_C,
-                                           lane, val);
-                } else {
-                  const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
-#pragma unroll
-                  for (int u = 0; u < kBEPL; ++u) {
-                    const float Ee = Ef[u] * z;
-                    val[u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
-                  }
-                }
-                accumulate(q, val);
-              }
+            for (int u = 0; u < kBEPL; ++u) {
+              P[u] += (double)pf[u];
+              pf[u] = 0.f;
             }
-            fold();
-            // drop the batch from the queue
-            const int rest = q_len - cnt;
-            const int moved = lane < rest ? s_q[cnt + lane] : 0;
-            __builtin_amdgcn_wave_barrier();
-            if (lane < rest) s_q[lane] = moved;
-            __builtin_amdgcn_wave_barrier();
-            return rest;
           };
-          int q_len = 0;
-          const int sweep_end = min(n, seg_hi);
-          for (int base = seg_lo & ~63;;) {
-            const bool more = base < sweep_end;
-            if (more) {  // queue the covered positions of the next 64
-              uint64_t cov = (uint64_t)__builtin_amdgcn_readfirstlane(s_cov[base >> 5]) |
-                             ((uint64_t)__builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]) << 32);
-              if (seg_lo > base) cov &= ~0ull << (seg_lo - base);
-              if (sweep_end - base < 64) cov &= (1ull << (sweep_end - base)) - 1;
-              const uint32_t c_lo = (uint32_t)cov, c_hi = (uint32_t)(cov >> 32);
-              const uint32_t below = __builtin_amdgcn_mbcnt_hi(c_hi, __builtin_amdgcn_mbcnt_lo(c_lo, 0u));
-              const bool mine = lane < 32 ? (c_lo >> lane) & 1u : (c_hi >> (lane - 32)) & 1u;
-              if (mine) s_q[q_len + (int)below] = base + lane;
-              __builtin_amdgcn_wave_barrier();
-              q_len += __builtin_popcount(c_lo) + __builtin_popcount(c_hi);
-              base += 64;
+          // in position order: window ends at or before pos use the
+          // prefix before F(pos); `due` marks the lanes whose position is
+          // at or after the next window end
+          auto accumulate = [&](int q, const float (&val)[kBEPL]) {
+            if (__builtin_expect((due >> q) & 1, 0)) {
+              const int pos = readlane_i(k, q);
+              fold();
+              while (ev_pos <= pos) fire();
+              due = __ballot(k >= ev_pos);
             }
-            if (q_len >= 64 || (!more && q_len > 0))
-              q_len = run_batch(min(q_len, 64), q_len);
-            else if (!more)
-              break;
-          }
-          while (ev < ev_end) fire();  // window ends past the last covered position
-          if (kBWP > 1) {
-            // segment totals through this workgroup's global scratch, then the
-            // offset of this segment (earlier segments in order) on its events
-            double* g_T = B.seg + (size_t)blockIdx.x * kBWP * kBEPL * 64;
-            if (wp < kBWP - 1)
 #pragma unroll
-              for (int u = 0; u < kBEPL; ++u) g_T[((size_t)wp * kBEPL + u) * 64 + lane] = P[u];
-            __syncthreads();
-            if (wp > 0) {
+            for (int u = 0; u < kBEPL; ++u) pf[u] += val[u];
+          };
+          // one patch at a time.  With the row-set path the slice rows of
+          // the next patch are read while the current one is evaluated (two
+          // row buffers, used in turn); the path is chosen once per batch
+          if (ATM == PPM_ATM_NSX && we.fast) {
+            auto step_rows = [&](const float (&p_cur)[kBSets][4], float (&p_next)[kBSets][4]) {
+              const int q = __builtin_ctzll(todo);
+              todo &= todo - 1;
+              if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_next);
+              float val[kBEPL];
+              if (__builtin_expect((interior >> q) & 1, 1))
+                nsx_eval_table<kBEPL, 0, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+              else if ((clamp_lo >> q) & 1)
+                nsx_eval_table<kBEPL, 3, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+              else
+                nsx_eval_table<kBEPL, 2, kBSets, EM>(le, pv, pr, q, ax, p_cur, lane, s_C, val);
+              accumulate(q, val);
+            };
+            float p_a[kBSets][4], p_b[kBSets][4];
+            if (todo) nsx_load_rows(pr, __builtin_ctzll(todo), s_S, lane, p_a);
+            while (todo) {
+              step_rows(p_a, p_b);
+              if (!todo) break;
+              step_rows(p_b, p_a);
+            }
+          } else {
+            while (todo) {
+              const int q = __builtin_ctzll(todo);
+              todo &= todo - 1;
+              float val[kBEPL];
+              if (ATM == PPM_ATM_NSX) {
+                float w[4];
 #pragma unroll
-              for (int u = 0; u < kBEPL; ++u) {
-                P[u] = 0.0;
-                for (int w = 0; w < wp; ++w) P[u] += g_T[((size_t)w * kBEPL + u) * 64 + lane];
-              }
-              ev = ev_begin;
-              win = ev_begin - 64;
-              load_ev();
-              while (ev < ev_end) {
-                if (ev_inside) {  // the window's other end is in this segment too
-                  ++ev;
-                  load_ev();
-                } else {
-                  fire();
+                for (int m = 0; m < 4; ++m) w[m] = readlane_f(pv.w[m], q);
+                nsx_eval_edge<kBEPL, EM>(le, asc, readlane_f(pv.tx, q), readlane_i(pv.im, q), w, ax, s_S, s_C,
+                                         lane, val);
+              } else {
+                const float c = readlane_f(pv.w[0], q), z = readlane_f(pv.zf, q);
+#pragma unroll
+                for (int u = 0; u < kBEPL; ++u) {
+                  const float Ee = Ef[u] * z;
+                  val[u] = c * (Ee * Ee * Ee / expm1f(Ee * inv_kT));
                 }
               }
+              accumulate(q, val);
             }
-            __syncthreads();  // the scratch is read before the next group writes it
           }
+          fold();
+          // drop the batch from the queue
+          const int rest = q_len - cnt;
+          const int moved = lane < rest ? s_q[cnt + lane] : 0;
+          __builtin_amdgcn_wave_barrier();
+          if (lane < rest) s_q[lane] = moved;
+          __builtin_amdgcn_wave_barrier();
+          return rest;
+        };
+        int q_len = 0;
+        const int sweep_end = min(nv, seg_hi);
+        for (int base = seg_lo & ~63;;) {
+          const bool more = base < sweep_end;
+          if (more) {  // queue the covered positions of the next 64
+            // (readfirstlane returns a signed int: widened through uint32_t,
+            // so a low word with bit 31 set does not fill the high word)
+            const uint32_t w_lo = (uint32_t)__builtin_amdgcn_readfirstlane(s_cov[base >> 5]);
+            const uint32_t w_hi = (uint32_t)__builtin_amdgcn_readfirstlane(s_cov[(base >> 5) + 1]);
+            uint64_t cov = (uint64_t)w_lo | ((uint64_t)w_hi << 32);
+            if (seg_lo > base) cov &= ~0ull << (seg_lo - base);
+            if (sweep_end - base < 64) cov &= (1ull << (sweep_end - base)) - 1;
+            const uint32_t c_lo = (uint32_t)cov, c_hi = (uint32_t)(cov >> 32);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(c_hi, __builtin_amdgcn_mbcnt_lo(c_lo, 0u));
+            const bool mine = lane < 32 ? (c_lo >> lane) & 1u : (c_hi >> (lane - 32)) & 1u;
+            if (mine) s_q[q_len + (int)below] = base + lane;
+            __builtin_amdgcn_wave_barrier();
+            q_len += __builtin_popcount(c_lo) + __builtin_popcount(c_hi);
+            base += 64;
+          }
+          if (q_len >= 64 || (!more && q_len > 0))
+            q_len = run_batch(min(q_len, 64), q_len);
+          else if (!more)
+            break;
+        }
+        while (ev < ev_end) fire();  // window ends past the last covered position
+        if (kBWP > 1) {
+          // segment totals through this workgroup's global scratch, then the
+          // offset of this segment (earlier segments in order) on its events
+          double* g_T = B.seg + (size_t)blockIdx.x * kBWP * kBEPL * 64;
+          if (wp < kBWP - 1)
+#pragma unroll
+            for (int u = 0; u < kBEPL; ++u) g_T[((size_t)wp * kBEPL + u) * 64 + lane] = P[u];
+          __syncthreads();
+          if (wp > 0) {
+#pragma unroll
+            for (int u = 0; u < kBEPL; ++u) {
+              P[u] = 0.0;
+              for (int w = 0; w < wp; ++w) P[u] += g_T[((size_t)w * kBEPL + u) * 64 + lane];
+            }
+            ev = ev_begin;
+            win = ev_begin - 64;
+            load_ev();
+            while (ev < ev_end) {
+              if (ev_inside) {  // the window's other end is in this segment too
+                ++ev;
+                load_ev();
+              } else {
+                fire();
+              }
+            }
+          }
+          __syncthreads();  // the scratch is read before the next group writes it
+        }
+        if (g1 < C.n_pts) {
+          g0 = g1;
+        } else {
+          ci = cj;
+          g0 = 0;
         }
       }
     }
@@ -623,8 +689,9 @@ __global__ __launch_bounds__(64 * kReduceWavesMax) void k_reduce_batched(BatchLa
 
 }  // namespace
 
-size_t batch_lds_bytes(int n_fine, int atmosphere, int nsx_rows, int cov_words, int ev_cap, bool fine_lds) {
-  return batch_layout(fine_lds ? n_fine + 1 : 0, atmosphere == PPM_ATM_NSX ? nsx_rows : 0, cov_words, ev_cap).total;
+size_t batch_lds_bytes(int n_fine, int atmosphere, int nm, int nsx_rows, int cov_words, int ev_cap, bool fine_lds) {
+  const bool nsx = atmosphere == PPM_ATM_NSX;
+  return batch_layout(fine_lds ? n_fine + 1 : 0, nsx ? nm : 0, nsx ? nsx_rows : 0, cov_words, ev_cap).total;
 }
 
 int batch_threads() { return kBThreads; }
@@ -670,7 +737,8 @@ int batched_prepare(int atmosphere, int epl, int emode, bool fine_lds, int lds_m
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, B.queue, B.n_evals);
-  const size_t lds = batch_lds_bytes(B.n_fine_max, B.atmosphere, B.nsx_rows, B.cov_words, B.ev_cap, B.fine_lds);
+  const size_t lds =
+      batch_lds_bytes(B.n_fine_max, B.atmosphere, N.nm, B.nsx_rows, B.cov_words, B.ev_cap, B.fine_lds);
   DevNsx n_arg = N;
   BatchLaunch b_arg = B;
   void* args[] = {&n_arg, &b_arg};
@@ -700,10 +768,24 @@ int launch_reduce_batched(const BatchLaunch& B, const BatchGrid& G, int s0, int 
   return hipSuccess;
 }
 
+// Resident workgroups per CU: the runtime's figure, limited by the LDS a
+// CU holds at the allocation granule measured on gfx950 (kLdsGranule): the
+// runtime counts finer granules, and a launch sized to its figure got 2 of
+// the 3 workgroups it reported (54064 bytes per workgroup ran at the speed
+// of 2 per CU, 53552 at 3).
+constexpr size_t kLdsGranule = 1280;
 int batched_occupancy(int atmosphere, int epl, int emode, bool fine_lds, size_t lds, int* wg_per_cu) {
-  int n = 0;
-  const hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_fn(atmosphere, epl, emode, fine_lds), kBThreads, lds);
+  const void* fn = batched_fn(atmosphere, epl, emode, fine_lds);
+  int n = 0, dev = 0, lds_cu = 0;
+  hipFuncAttributes fa{};
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kBThreads, lds);
+  if (e == hipSuccess) e = hipFuncGetAttributes(&fa, fn);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+  if (e == hipSuccess) {
+    const size_t per_wg = (lds + fa.sharedSizeBytes + kLdsGranule - 1) / kLdsGranule * kLdsGranule;
+    n = std::min(n, (int)((size_t)lds_cu / per_wg));
+  }
   *wg_per_cu = n;
   return e;
 }

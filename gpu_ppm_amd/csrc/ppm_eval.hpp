@@ -31,6 +31,62 @@ __device__ inline int vec_hunt(const T* xx, int n, double x, double& a) {
   return i;
 }
 
+// Bucket index of an ascending table for VecHunt look-ups: the search of x
+// starts from the brackets of the two bucket bounds around x instead of the
+// whole table.  idx[b] is VecHunt's bracket of z_b = x0 + b / scale,
+// b = 0..kHuntBuckets, with x0 = xx[0] and scale = kHuntBuckets /
+// (xx[n-1] - xx[0]).  For xx[lo] <= x < xx[hi] (or hi = n - 1) the bracket
+// is the largest i <= n - 2 with xx[i] <= x, which a binary search between
+// lo and hi finds as the full search does.  The bounds are taken one
+// millionth of a bucket outside x, far beyond the rounding of z_b and of the
+// bucket of x; below the first bucket the search starts at node 0 and beyond
+// the last it ends at node n - 1, so that queries outside the table (and
+// tables with repeated end nodes) give the full search's bracket too.  The
+// bracket and the fraction equal the full search's bit for bit.
+// idx == nullptr: full search (a descending or degenerate table).
+constexpr int kHuntBuckets = 64;
+struct HuntIndex {
+  const uint16_t* idx;
+  double x0, scale;
+};
+
+template <typename T>
+__device__ inline int hunt(const T* xx, int n, double x, const HuntIndex& H, double& a) {
+  if (H.idx == nullptr) return vec_hunt(xx, n, x, a);
+  const double t = fmin(fmax((x - H.x0) * H.scale, -1.0), (double)kHuntBuckets + 1.0);
+  const double tl = t - 1e-6;  // below the first bucket: from node 0, as the full search
+  int jl = tl < 0 ? 0 : H.idx[clampi((int)floor(tl), 0, kHuntBuckets)];
+  const int bh = (int)floor(t + 1e-6) + 1;  // beyond the last bucket: to node n - 1
+  int ju = bh > kHuntBuckets ? n - 1 : min((int)H.idx[max(bh, 0)] + 1, n - 1);
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) >> 1;
+    if (x >= (double)xx[jm])
+      jl = jm;
+    else
+      ju = jm;
+  }
+  const int i = clampi(jl, 0, n - 2);
+  a = (x - (double)xx[i]) / ((double)xx[i + 1] - (double)xx[i]);
+  return i;
+}
+
+// Builds the bucket index of xx[0..n-1] into idx (kHuntBuckets + 1 entries,
+// threads tid, tid + nthreads, ...) and returns its HuntIndex; the caller
+// synchronises before the first look-up.  Tables that do not ascend, or
+// whose nodes do not fit 16 bits, get the full search.
+template <typename T>
+__device__ inline HuntIndex build_hunt_index(const T* xx, int n, uint16_t* idx, int tid, int nthreads) {
+  if (n < 2 || n > 65535) return HuntIndex{nullptr, 0.0, 0.0};
+  const double x0 = (double)xx[0], span = (double)xx[n - 1] - x0;
+  if (!(span > 0)) return HuntIndex{nullptr, 0.0, 0.0};
+  const double scale = kHuntBuckets / span;
+  for (int b = tid; b <= kHuntBuckets; b += nthreads) {
+    double a;
+    idx[b] = (uint16_t)vec_hunt(xx, n, x0 + b / scale, a);
+  }
+  return HuntIndex{idx, x0, scale};
+}
+
 // A ring's fine-phase table staged in LDS.
 struct RingTab {
   const double* ph;  // time-delayed phase, [nf1]
@@ -65,22 +121,27 @@ struct TxMap {
 
 // flux/I of the patch at rotational phase pp, with the fine-table bracket
 // (i, a) it was interpolated in.  Zero exactly where the patch is skipped.
-__device__ inline double patch_flux_factor(double pp, const RingTab& R, int& i, double& a) {
-  i = vec_hunt(R.ph, R.nf1, pp, a);
+// PH: bucket index of the ring's phase table (or the full search).
+__device__ inline double patch_flux_factor(double pp, const RingTab& R, int& i, double& a,
+                                           const HuntIndex& PH = HuntIndex{nullptr, 0.0, 0.0}) {
+  i = hunt(R.ph, R.nf1, pp, PH, a);
   return (1 - a) * (double)R.fI[i] + a * (double)R.fI[i + 1];
 }
 
+// PH, MU: bucket indexes of the ring's phase table and of the mu nodes
+// (their brackets equal the full search's).
 template <int ATM>
 __device__ inline void patch_setup(double pp, double weight, const RingTab& R, const double* s_mu,
                                    const double* mu_den, int nm, double mu_min, int mu_mode, TxMap txm,
-                                   PatchVals& o) {
+                                   PatchVals& o, const HuntIndex& PH = HuntIndex{nullptr, 0.0, 0.0},
+                                   const HuntIndex& MU = HuntIndex{nullptr, 0.0, 0.0}) {
   o.tx = 0.f;
   o.w[0] = o.w[1] = o.w[2] = o.w[3] = 0.f;
   o.zf = 0.f;
   o.im = -1;
   double a;
   int i;
-  const double fI = patch_flux_factor(pp, R, i, a);
+  const double fI = patch_flux_factor(pp, R, i, a, PH);
   if (fI == 0) return;
   const double zz = (1 - a) * (double)R.zf[i] + a * (double)R.zf[i + 1];
   const double c = fI * weight;
@@ -89,7 +150,7 @@ __device__ inline void patch_setup(double pp, double weight, const RingTab& R, c
     double mu = (1 - a) * (double)R.cs[i] + a * (double)R.cs[i + 1];
     if (mu < mu_min) mu = mu_min;
     double am;
-    int im = vec_hunt(s_mu, nm, mu, am);
+    int im = hunt(s_mu, nm, mu, MU, am);
     if (mu_mode == kMuLinear || (mu_mode == kMuCubicLinearEdge && im >= nm - 5)) {  // linear
       o.w[0] = (float)(c * (1 - am));
       o.w[1] = (float)(c * am);
