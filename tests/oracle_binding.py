@@ -161,3 +161,67 @@ def rel_err(x, ref, floor=1e-6):
     if not m.any():
         return 0.0
     return float(np.max(np.abs(x[m] - ref[m]) / np.abs(ref[m])))
+
+
+def distinct_phases(star, grid):
+    """Distinct rotational phases among the reference's (phase point, active
+    patch) pairs, summed over rings and spots.
+
+    The reference evaluates every phase point p at every active patch j of a
+    ring at the rotational phase frac(p + (j - s/2) / n)
+    (cpu/devarshi24st_cpu.cpp:214-239); pairs of one ring with equal phases
+    need one evaluation.  Two pairs share a phase exactly when their
+    fractional offsets frac(p n) agree and (floor(p n) + j) mod n agree; the
+    offsets are compared within 1e-9 and floor(p n) is taken up when the
+    fraction exceeds 1 - 1e-12, as for the phase points of the batched path.
+    Membership is the oracle's FP64 cap test (cpu/grid.hpp:144-148) on its
+    HEALPix pixel centres.  Caps (PPM_MEMBER_CAP) with GL or uniform
+    sampling; every patch counts, visible or not."""
+    import math
+    L = lib()
+    ns, npp = grid.n_side, grid.n_phase
+    if grid.sampling == abi.SAMPLE_GL:
+        pts = [L.or_gl_node(grid.n_gl, k, b / npp, (b + 1) / npp) for b in range(npp) for k in range(grid.n_gl)]
+    else:
+        pts = [b / npp for b in range(npp)]
+    z, n_pix, s = c_double(), ctypes.c_int(), c_double()
+    total = 0
+    for q in range(star.n_spots):
+        sp = star.spot[q]
+        if sp.membership != abi.MEMBER_CAP:
+            raise ValueError("caps only")
+        for i in range(1, 4 * ns):
+            L.or_healpix_ring(ns, i, byref(z), byref(n_pix), byref(s))
+            n, zz = n_pix.value, z.value
+            s2 = math.sqrt(1.0 - zz * zz)
+            active = []
+            for j in range(1, n + 1):
+                phi = L.or_healpix_phi(ns, i, j)
+                inside = math.sin(sp.theta1) * s2 * math.cos(phi - sp.phi1) + math.cos(sp.theta1) * zz > math.cos(sp.angrad1)
+                if inside and sp.angrad2 > 0:
+                    inside = not (math.sin(sp.theta2) * s2 * math.cos(phi - sp.phi2) + math.cos(sp.theta2) * zz
+                                  > math.cos(sp.angrad2))
+                if inside:
+                    active.append(j)
+            if not active:
+                continue
+            offs = []
+            for p in pts:
+                v = p * n
+                f = math.floor(v)
+                d = v - f
+                if d > 1.0 - 1e-12:
+                    d, f = 0.0, f + 1
+                offs.append((d, int(f)))
+            # classes: offsets within 1e-9 of the class's first, in ascending order
+            cls, first = {}, None
+            for d in sorted(set(o[0] for o in offs)):
+                if first is None or d - first > 1e-9:
+                    first = d
+                cls[d] = first
+            act = np.array(active)
+            seen = set()
+            for d, f in offs:
+                seen.update((cls[d], int(k)) for k in (f + act) % n)
+            total += len(seen)
+    return total
