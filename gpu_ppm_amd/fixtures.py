@@ -14,6 +14,7 @@ Layout under a root directory (what the drop-in programs read):
                                      axes (log g 13.7-14.7), seed 1
 """
 import os
+import shutil
 import subprocess
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -28,25 +29,39 @@ def _synth(*args):
     subprocess.run([SYNTH] + [str(a) for a in args], check=True)
 
 
+def _publish(path, make):
+    """Runs make(tmp) on a private temporary path and renames the result to
+    path, so that processes generating the same input at once (pytest
+    workers) never read a half-written file or directory."""
+    if os.path.exists(path):
+        return
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = "%s.tmp%d" % (path, os.getpid())
+    make(tmp)
+    try:
+        os.rename(tmp, path)
+    except OSError:  # another process published it first
+        if not os.path.exists(path):
+            raise
+        if os.path.isdir(tmp):
+            shutil.rmtree(tmp)
+        else:
+            os.remove(tmp)
+
+
 def ensure_lensing(root, setting="std"):
     d = os.path.join(root, "tables", "lensing", setting)
-    if not os.path.exists(os.path.join(d, "cdt_over_R_of_u_cos_alpha.txt")):
-        if setting == "std":
-            _synth("lensing", d, 0.3, 0.5, 256, 512)
-        elif setting == "wide":
-            _synth("lensing", d, 0.15, 0.55, 256, 512)
-        elif setting == "coarse":
-            _synth("lensing", d, 0.3, 0.5, 64, 128)
-        else:
-            raise ValueError(setting)
+    args = {"std": (0.3, 0.5, 256, 512), "wide": (0.15, 0.55, 256, 512), "coarse": (0.3, 0.5, 64, 128)}
+    if setting not in args:
+        raise ValueError(setting)
+    _publish(d, lambda t: _synth("lensing", t, *args[setting]))
     return d
 
 
 def ensure_response(root, setting="std"):
     d = os.path.join(root, "tables", "ism_inst", setting)
-    if not os.path.exists(os.path.join(d, "rsp.txt")):
-        sizes = {"std": (300, 360), "std128": (300, 128), "ref270": (270, 128)}[setting]
-        _synth("response", d, sizes[0], sizes[1], 0)
+    sizes = {"std": (300, 360), "std128": (300, 128), "ref270": (270, 128)}[setting]
+    _publish(d, lambda t: _synth("response", t, sizes[0], sizes[1], 0))
     return d
 
 
@@ -57,11 +72,10 @@ def nsx_path(root, small=False):
 
 def ensure_nsx(root, small=False):
     p = nsx_path(root, small)
-    if not os.path.exists(p):
-        if small:
-            _synth("nsx", p, 1, *SMALL_NSX_DIMS)
-        else:
-            _synth("nsx", p, 1)
+    if small:
+        _publish(p, lambda t: _synth("nsx", t, 1, *SMALL_NSX_DIMS))
+    else:
+        _publish(p, lambda t: _synth("nsx", t, 1))
     return p
 
 
@@ -74,8 +88,7 @@ def ensure_nsx_bb(root):
     filled with the isotropic Planck intensity alone: logInuT3 =
     log10(2 k^3 x^3 / (h^2 c^2 (e^x - 1))), x = 10^logEkT."""
     p = nsx_bb_path(root)
-    if not os.path.exists(p):
-        _synth("nsx_bb", p, *SMALL_NSX_DIMS)
+    _publish(p, lambda t: _synth("nsx_bb", t, *SMALL_NSX_DIMS))
     return p
 
 
@@ -83,8 +96,7 @@ def ensure_nsx_v171019(root):
     """NSX-format grid on the nsx_H_v171019 axes (35x11x166x67, log g
     13.7-14.7, cpu/nsx.hpp:39-51), seed 1."""
     p = os.path.join(root, "ext", "model_data", "nsx_H_v171019.out")
-    if not os.path.exists(p):
-        _synth("nsx", p, 1, 35, 11, 166, 67, 14.7)
+    _publish(p, lambda t: _synth("nsx", t, 1, 35, 11, 166, 67, 14.7))
     return p
 
 

@@ -48,8 +48,7 @@ def golden_root():
     root = os.path.join(DATA, "golden_inputs")
     fixtures.ensure_lensing(root, "std")
     d = os.path.join(root, "tables", "ism_inst", "std")
-    if not os.path.exists(os.path.join(d, "rsp.txt")):
-        fixtures._synth("response", d, 270, 128, 0)
+    fixtures._publish(d, lambda t: fixtures._synth("response", t, 270, 128, 0))
     fixtures.ensure_nsx(root, small=False)
     with open(os.path.join(TESTS, "golden", "ref_cpu", "INPUTS.sha256")) as f:
         for line in f:
