@@ -458,6 +458,36 @@ def bench_plain(root, steps, warmup):
                 n_eval_ref=st[0].n_eval_ref, n_active=st[0].n_active)
 
 
+def phase_breakdown(root, n_stars):
+    """Wavefront time of k_integral_batched by phase (libppm_phase.so: the
+    same kernels with shader-clock reads at the phase boundaries), for one
+    direct launch of the batched workload after an untimed one.  Fractions
+    are of the summed wavefront time; the sweep includes the patch set-up
+    and the window ends, and "evaluation" is the rest of the sweep."""
+    from gpu_ppm_amd import abi, hip
+    from gpu_ppm_amd import scenarios as S
+    from gpu_ppm_amd.lib import Context
+    os.environ["PPM_NO_GRAPH"] = "1"
+    stars = S.sample_priors(n_stars, seed=0)
+    ctx = Context(root, lensing="wide", responses=("std",))
+    n_ch, _ = ctx.response_info(0)
+    d_flux = hip.DeviceArray((n_stars, N_PHASE, N_E))
+    d_cnt = hip.DeviceArray((n_stars, n_ch, N_PHASE))
+    gb = grid(abi.METHOD_BATCHED)
+    ctx.eval_device(stars, gb, d_flux.ptr, d_cnt.ptr)
+    ctx.phase_timers(reset=True)
+    st = ctx.eval_device(stars, gb, d_flux.ptr, d_cnt.ptr)
+    ph = ctx.phase_timers()
+    ctx.close()
+    tot = float(ph["total"])
+    frac = {k: ph[k] / tot for k in abi.PHASE_NAMES if k != "total"}
+    frac["evaluation"] = (ph["sweep"] - ph["patch_setup"] - ph["window_ends"]) / tot
+    return {"kernel": "k_integral_batched", "library": "libppm_phase.so", "n_stars": n_stars,
+            "integral_ms": st.integral_ms, "wavefronts": ph["wavefronts"], "cycles_total": ph["total"],
+            "fraction_of_wavefront_time": frac,
+            "workload": "batched: %d prior-drawn stars, N_side 1024, 64 bins x 2 GL, 360 E, NSX" % n_stars}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,6 +504,9 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=192, help="requests in the capacity burst")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
+    ap.add_argument("--phase-timers", action="store_true",
+                    help="only report where k_integral_batched's wavefronts spend their time, from the "
+                         "phase-timer build of the library (libppm_phase.so), on the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (profiles shard by star or by "
@@ -481,6 +514,10 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
+    if args.phase_timers:
+        os.environ["PPM_LIBRARY"] = "phase"  # before the library is loaded
+        print(json.dumps(phase_breakdown(data_root(), args.stars)), flush=True)
         return
     root = data_root()
     log("bench: data ready; workload", args.workload)

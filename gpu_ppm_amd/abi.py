@@ -40,6 +40,12 @@ METHOD_AUTO = 0
 METHOD_PLAIN = 1
 METHOD_BATCHED = 2
 
+# phases of the batched integral reported by ppm_phase_timers, in order (the
+# sweep includes the patch set-up and the window ends)
+PHASE_NAMES = ("fetch", "ring_setup", "events", "sort", "visible", "split", "sweep", "patch_setup",
+               "window_ends", "offsets", "total")
+PHASE_COUNT = 12  # PPM_PHASE_COUNT: the phases and the wavefront count
+
 # physical constants shared by the scenario builders (gpu/unit.cuh:3-22)
 KB_KEV_PER_K = 8.617333262e-8
 KPC_KM = 3.08567758e16

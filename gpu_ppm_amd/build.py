@@ -15,6 +15,8 @@ CSRC = os.path.join(PKG, "csrc")
 BIN = os.path.join(PKG, "_bin")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libppm.so")
+PHASE_LIB = os.path.join(PKG, "libppm_phase.so")
+PHASE_SOURCE = "ppm_batch_kernels.hip"  # the one object that differs in libppm_phase.so
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PPM_OFFLOAD_ARCH", "gfx950")
@@ -43,22 +45,27 @@ def build_synth():
 
 
 def build_lib():
+    """libppm.so, and libppm_phase.so: the same library whose batched
+    integral carries phase timers (ppm_phase_timers; PPM_LIBRARY=phase
+    loads it)."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = [(s, ()) for s in srcs] + [(PHASE_SOURCE, ("-DPPM_PHASE_TIMERS=1",))]
 
-    def one(src):
-        obj = os.path.join(OBJ, src + ".o")
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if not src.endswith(".hip"):
-            cmd = [HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd)
+    def one(job):
+        src, defs = job
+        obj = os.path.join(OBJ, src + (".phase" if defs else "") + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC] + HIP_FLAGS + list(defs) + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(one, srcs))
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"])
-    os.replace(tmp, LIB)
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(one, jobs))
+    plain, phase = objs[:len(srcs)], [o if s != PHASE_SOURCE else objs[-1] for o, s in zip(objs, srcs)]
+    for out, ob in ((LIB, plain), (PHASE_LIB, phase)):
+        tmp = out + ".tmp"
+        _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + ob + ["-lpthread"])
+        os.replace(tmp, out)
 
 
 def build_apps():

@@ -1029,4 +1029,15 @@ ppm_status ppm_get_launch_shape(ppm_ctx* c, const ppm_grid* grid, int n_stars, i
   return PPM_OK;
 }
 
+ppm_status ppm_phase_timers(ppm_ctx* c, uint64_t* cycles, int n, int reset) {
+  if (!c || !cycles || n < 1 || n > PPM_PHASE_COUNT) return PPM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->exec_mu);
+  hipSetDevice(c->device);
+  if (hipDeviceSynchronize() != hipSuccess) return fail(c, PPM_ERR_DEVICE, "device synchronisation failed");
+  static_assert(sizeof(uint64_t) == sizeof(unsigned long long), "cycle counters are 64-bit");
+  if (!batch_phase_timers(reinterpret_cast<unsigned long long*>(cycles), n, reset != 0))
+    return fail(c, PPM_ERR_UNSUPPORTED, "this library is built without phase timers (libppm_phase.so has them)");
+  return PPM_OK;
+}
+
 }  // extern "C"

@@ -91,6 +91,9 @@ size_t batch_lds_bytes(int n_fine, int atmosphere, int nm, int nsx_rows, int cov
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
 size_t batch_seg_doubles();  // workgroup scratch of the segment totals
+// Phase-timer totals of the batched integral (a build with PPM_PHASE_TIMERS=1;
+// false otherwise): n of the kNumPhases totals and the wavefront count.
+bool batch_phase_timers(unsigned long long* cycles, int n, bool reset);
 int batch_energies(int epl);  // energies per item
 int batched_prepare(int atmosphere, int epl, int emode, bool fine_lds, int lds_max);  // outside any stream capture
 int launch_batched(const DevNsx& N, const BatchLaunch& B, int n_wg, void* stream);

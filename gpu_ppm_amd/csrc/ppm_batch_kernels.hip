@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 
 #include "ppm_batch_dev.hpp"
 #include "ppm_eval.hpp"
@@ -24,6 +25,56 @@
 namespace ppm {
 
 namespace {
+
+// Phase timers (a build with PPM_PHASE_TIMERS=1 only): each wavefront adds
+// the shader-clock cycles it spends in each phase of the integral and posts
+// its totals once at the end; the default build compiles them out.
+#ifndef PPM_PHASE_TIMERS
+#define PPM_PHASE_TIMERS 0
+#endif
+enum BatchPhase {
+  kPhFetch,     // taking the next work item (two barriers and the queue)
+  kPhStage,     // ring set-up: fine tables and NSX slice rows into LDS
+  kPhEvents,    // windows -> events and the coverage bitmap
+  kPhSort,      // rank sort of the events
+  kPhVisible,   // dropping covered positions whose patch is not seen
+  kPhSplit,     // the split into segments and their events
+  kPhSweep,     // the sweep (includes the next two)
+  kPhSetup,     //   FP64 patch set-up of each batch of 64 positions
+  kPhEnds,      //   window ends posted during the sweep
+  kPhOffsets,   // segment totals and offsets (both barriers)
+  kPhTotal,     // the whole kernel
+  kNumPhases
+};
+static_assert(kNumPhases + 1 == PPM_PHASE_COUNT, "ppm_phase_timers reports every phase and the wavefront count");
+#if PPM_PHASE_TIMERS
+__device__ unsigned long long g_phase[kNumPhases + 1];  // + the number of wavefronts
+#define PT_DECL unsigned long long pt_[kNumPhases] = {}, pt_t = clock64(), pt_0 = pt_t;
+#define PT_MARK() (pt_t = clock64())
+#define PT_ADD(ph)                 \
+  do {                             \
+    const unsigned long long t_ = clock64(); \
+    pt_[ph] += t_ - pt_t;          \
+    pt_t = t_;                     \
+  } while (0)
+#define PT_SCOPE_BEGIN(v) const unsigned long long v = clock64();
+#define PT_SCOPE_END(v, ph) (pt_[ph] += clock64() - (v))
+#define PT_POST()                                                              \
+  do {                                                                         \
+    pt_[kPhTotal] = clock64() - pt_0;                                          \
+    if (lane == 0) {                                                           \
+      for (int i_ = 0; i_ < kNumPhases; ++i_) atomicAdd(&g_phase[i_], pt_[i_]); \
+      atomicAdd(&g_phase[kNumPhases], 1ull);                                   \
+    }                                                                          \
+  } while (0)
+#else
+#define PT_DECL
+#define PT_MARK() ((void)0)
+#define PT_ADD(ph) ((void)0)
+#define PT_SCOPE_BEGIN(v)
+#define PT_SCOPE_END(v, ph) ((void)0)
+#define PT_POST() ((void)0)
+#endif
 
 constexpr int kBW = 4;                  // wavefronts per workgroup, each a segment of the positions
 constexpr int kBWP = kBW;
@@ -188,13 +239,16 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
   const double h = (N.logE_max - N.logE_min) / (N.nE - 1);
   const double inv_h = 1.0 / h;
   unsigned int n_done = 0;  // evaluations by this lane
+  PT_DECL
 
   for (int round = 0;; ++round) {
+    PT_MARK();
     __syncthreads();
     if (tid == 0) s_item = B.persistent ? atomicAdd(B.queue, 1) : (round == 0 ? (int)blockIdx.x : B.n_items);
     __syncthreads();
     const int q_idx = s_item;
     if (q_idx >= B.n_items) break;
+    PT_ADD(kPhFetch);
     const BatchItem It = B.items[B.order[q_idx]];
     const BatchGrid G = B.grids[It.grid];
     const int n_e = G.n_e, n_phase = G.n_phase, nf1 = G.n_fine + 1;
@@ -229,6 +283,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       if (!Dp->kept) continue;
       const RingTask T = B.tasks[task];
       const SpotParams& S = B.spots[T.spot];
+      PT_MARK();
       __syncthreads();  // previous ring's LDS is no longer read
       const double* gph = G.ph + (size_t)(task - G.task0) * nf1;
       const float* gf = G.fine + (size_t)(task - G.task0) * 3 * nf1;
@@ -259,6 +314,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         stage_nsx_rows(N, *Dp, r_lo, n_rows, s_S, tid, kBThreads);
       }
       __syncthreads();
+      PT_ADD(kPhStage);
       const TxMap txm{(float)((l_ref - log10(S.kT_nsx) - N.logE_min) * inv_h - r_lo),
                       (float)(inv_h * 0.30102999566398120)};
       const float inv_kT = (float)S.inv_kT;
@@ -317,6 +373,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           return pp;
         };
         // ---- windows -> events and coverage
+        PT_MARK();
         __syncthreads();
         for (int w = tid; w < n_words; w += kBThreads) s_cov[w] = 0u;
         if (tid == 0) s_nev = 0;
@@ -364,6 +421,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         }
         __syncthreads();
         const int n_ev = __builtin_amdgcn_readfirstlane(s_nev);
+        PT_ADD(kPhEvents);
         // ---- rank sort of the events (keys are unique) into records.  The
         // keys 2p, 2p + 1 are the two ends of one window piece: a thread
         // ranks both in one pass and keeps its first pair's ranks and
@@ -388,6 +446,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             pr_p1 = p1;
           }
         }
+        PT_ADD(kPhSort);
         // ---- keep the covered positions whose patch is seen (flux/I != 0):
         // the others add nothing, so the split, the queue and the set-up
         // only count patches that contribute.  Each wavefront rewrites its
@@ -407,6 +466,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         }
         __syncthreads();
 
+        PT_ADD(kPhVisible);
         // ---- sweep.  The covered positions are split in kBWP segments of
         // equal count, [seg_lo, seg_hi) for this wavefront; it takes the
         // events in its segment with a prefix that starts at zero, and the
@@ -437,6 +497,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         }
         const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
         const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
+        PT_ADD(kPhSplit);
         double P[kBEPL];
 #pragma unroll
         for (int u = 0; u < kBEPL; ++u) P[u] = 0.0;
@@ -470,17 +531,20 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           }
         };
         auto fire = [&]() {
+          PT_SCOPE_BEGIN(pt_f)
           double* const row = acc + lane + ev_bin * en;  // the energies' offsets are immediates
 #pragma unroll
           for (int u = 0; u < kBEPL; ++u)
             if (u * 64 + lane <= e_lim) unsafeAtomicAdd(row + u * 64, ev_sw * P[u]);
           ++ev;
           load_ev();
+          PT_SCOPE_END(pt_f, kPhEnds);
         };
         load_ev();
         // covered positions, in ascending order, 64 per batch: one patch
         // set-up per lane, then every lane evaluates each visible patch
         auto run_batch = [&](int cnt, int q_len) {
+          PT_SCOPE_BEGIN(pt_s)
           const int k = lane < cnt ? s_q[lane] : 0;  // virtual position
           PatchVals pv;
           pv.im = -1;
@@ -527,6 +591,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
 #pragma unroll
             for (int u = 0; u < kBEPL; ++u) pf[u] += val[u];
           };
+          PT_SCOPE_END(pt_s, kPhSetup);
           // one patch at a time.  With the row-set path the slice rows of
           // the next patch are read while the current one is evaluated (two
           // row buffers, used in turn); the path is chosen once per batch
@@ -608,6 +673,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             break;
         }
         while (ev < ev_end) fire();  // window ends past the last covered position
+        PT_ADD(kPhSweep);
         if (kBWP > 1) {
           // segment totals through this workgroup's global scratch, then the
           // offset of this segment (earlier segments in order) on its events
@@ -635,6 +701,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             }
           }
           __syncthreads();  // the scratch is read before the next group writes it
+          PT_ADD(kPhOffsets);
         }
         if (g1 < C.n_pts) {
           g0 = g1;
@@ -645,6 +712,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       }
     }
   }
+  PT_POST();
   // evaluations performed (statistics)
   unsigned long long tot = n_done;
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
@@ -695,6 +763,24 @@ size_t batch_lds_bytes(int n_fine, int atmosphere, int nm, int nsx_rows, int cov
 }
 
 int batch_threads() { return kBThreads; }
+
+bool batch_phase_timers(unsigned long long* cycles, int n, bool reset) {
+#if PPM_PHASE_TIMERS
+  unsigned long long h[kNumPhases + 1];
+  if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_phase), sizeof(h)) != hipSuccess) return false;
+  std::memcpy(cycles, h, sizeof(unsigned long long) * std::min(n, kNumPhases + 1));
+  if (reset) {
+    std::memset(h, 0, sizeof(h));
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase), h, sizeof(h)) != hipSuccess) return false;
+  }
+  return true;
+#else
+  (void)cycles;
+  (void)n;
+  (void)reset;
+  return false;
+#endif
+}
 int batch_acc_copies() { return kBWP; }
 size_t batch_seg_doubles() { return (size_t)kBWP * kMaxEPL * 64; }
 int batch_energies(int epl) { return 64 * epl; }

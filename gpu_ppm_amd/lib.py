@@ -17,6 +17,9 @@ from . import abi
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libppm.so")
+# PPM_LIBRARY=phase selects the build with phase timers in the batched
+# integral (ppm_phase_timers); it computes the same results.
+PHASE_LIB_PATH = os.path.join(PKG, "libppm_phase.so")
 
 
 class PpmError(RuntimeError):
@@ -34,9 +37,10 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libppm.so is not built; run __graft_entry__.build()")
-    L = ctypes.CDLL(LIB_PATH)
+    path = PHASE_LIB_PATH if os.environ.get("PPM_LIBRARY") == "phase" else LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built; run __graft_entry__.build()" % os.path.basename(path))
+    L = ctypes.CDLL(path)
     L.ppm_abi_version.restype = c_int
     L.ppm_strerror.restype = c_char_p
     L.ppm_strerror.argtypes = [c_int]
@@ -86,6 +90,8 @@ def load():
                                     POINTER(c_double)]
     L.ppm_response_build.restype = c_int
     L.ppm_response_build.argtypes = [POINTER(abi.ResponseSpec), POINTER(c_double), POINTER(c_double)]
+    L.ppm_phase_timers.restype = c_int
+    L.ppm_phase_timers.argtypes = [c_void_p, POINTER(c_uint64), c_int, c_int]
     _lib = L
     return L
 
@@ -279,6 +285,18 @@ class Context:
         sh = abi.LaunchShape()
         self._check(load().ppm_get_launch_shape(self._h, byref(grid), n_stars, atmosphere, byref(sh)))
         return sh.items_per_slot, sh.wg_per_cu, sh.energies_per_lane
+
+    def phase_timers(self, reset=False):
+        """Wavefront-cycle totals of the batched integral's phases since the
+        last reset, as a dict (abi.PHASE_NAMES, plus "wavefronts"); raises
+        PpmError (PPM_ERR_UNSUPPORTED) unless the library is the
+        phase-timer build (PPM_LIBRARY=phase)."""
+        n = len(abi.PHASE_NAMES) + 1
+        buf = (c_uint64 * n)()
+        self._check(load().ppm_phase_timers(self._h, buf, n, 1 if reset else 0))
+        out = dict(zip(abi.PHASE_NAMES, (int(v) for v in buf[:n - 1])))
+        out["wavefronts"] = int(buf[n - 1])
+        return out
 
     # -- serving
     def submit(self, star, grid):

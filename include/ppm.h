@@ -363,6 +363,19 @@ ppm_status ppm_set_launch_shape(ppm_ctx* ctx, const ppm_grid* grid, int n_stars,
 ppm_status ppm_get_launch_shape(ppm_ctx* ctx, const ppm_grid* grid, int n_stars, int atmosphere,
                                 ppm_launch_shape* shape);
 
+/* Phase timers of the batched integral.  No counterpart in the reference
+ * (it has no instrumentation); for profiling.  In a library built with them
+ * (python -m gpu_ppm_amd.build --phase-timers writes libppm_phase.so, the
+ * same kernels with shader-clock reads at phase boundaries), copies the
+ * wavefront-cycle totals accumulated since the last reset into
+ * cycles[0..n-1]: taking work items, ring set-up, events, sort, visibility
+ * pass, segment split, sweep (which includes the next two), FP64 patch
+ * set-up, window ends, segment offsets, whole kernel, and the number of
+ * wavefronts; reset != 0 clears them.  Other builds return
+ * PPM_ERR_UNSUPPORTED. */
+#define PPM_PHASE_COUNT 12
+ppm_status ppm_phase_timers(ppm_ctx* ctx, uint64_t* cycles, int n, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -153,3 +153,13 @@ def test_fit_entry_point_rejects_bad_arguments():
     data = (ctypes.c_double * 4)()
     assert L.ppm_eval_fit(None, ctypes.byref(st), 1, ctypes.byref(g), data, 0, ctypes.byref(fit), None,
                           None) == abi.ERR_ARG
+
+
+def test_phase_timer_build_has_the_same_entry_points():
+    """libppm_phase.so (the batched integral with phase timers) exports every
+    declared symbol too, and the phase count agrees with PPM_PHASE_COUNT."""
+    assert os.path.exists(lib.PHASE_LIB_PATH)
+    out = subprocess.run(["nm", "-D", "--defined-only", lib.PHASE_LIB_PATH], capture_output=True, text=True).stdout
+    assert set(declared_functions()) <= set(re.findall(r" T (ppm_\w+)", out))
+    n = int(re.search(r"#define PPM_PHASE_COUNT (\d+)", open(HEADER).read()).group(1))
+    assert n == abi.PHASE_COUNT == len(abi.PHASE_NAMES) + 1
