@@ -324,7 +324,9 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
       // read after the first class group's barrier
       const HuntIndex PH = build_hunt_index(R.ph, nf1, s_ph_idx, tid, kBThreads);
       const bool point = T.ring == 0;
-      const bool ring_hidden = Dp->hidden != 0;
+      const bool ring_hidden = Dp->hidden != kHiddenNone;
+      const bool gap_interval = Dp->hidden == kHiddenInterval;
+      const double gap_lo = Dp->gap_lo, gap_hi = Dp->gap_hi;
       const int n = point ? 1 : Dp->n_pix;
       const double inv_n = 1.0 / n;
       const int n_runs = point ? 1 : Dp->n_runs;
@@ -456,9 +458,14 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           const int v = base + lane, s = base / n_pad, k = v - s * n_pad;
           bool keep = false;
           if (k < n && ((s_cov[v >> 5] >> (v & 31)) & 1u)) {
-            int fi;
-            double fa;
-            keep = patch_flux_factor(phase_of(k, s), R, fi, fa, PH) != 0;
+            const double pp = phase_of(k, s);
+            if (gap_interval) {  // the set-up skips the rare kept patch whose flux/I rounds to 0
+              keep = !(pp >= gap_lo && pp <= gap_hi);
+            } else {
+              int fi;
+              double fa;
+              keep = patch_flux_factor(pp, R, fi, fa, PH) != 0;
+            }
           }
           const uint64_t m = __ballot(keep);
           if (lane == 0) s_cov[base >> 5] = (uint32_t)m;

@@ -64,6 +64,12 @@ struct RingTask {
 };
 
 // Output of the setup kernel, one per RingTask.
+// RingDesc::hidden: every fine node visible; some invisible (flux/I = 0
+// there, the patch skip of cpu/devarshi24st_cpu.cpp:226 applies); or some
+// invisible and the zeros of the interpolated flux/I are the phase interval
+// [gap_lo, gap_hi] (one gap of invisible nodes, ascending phase table).
+enum { kHiddenNone = 0, kHiddenLookup = 1, kHiddenInterval = 2 };
+
 struct RingDesc {
   double z, d_omega, logg;
   double zf_min, zf_max;      // range of the redshift factor over fine nodes
@@ -74,7 +80,8 @@ struct RingDesc {
   int flags;                  // bit0: run overflow, bit1: NSX rows beyond the staged slice
   int iT, ig;                 // first (logT, log g) node of the NSX stencil
   int n_tg;                   // stencil nodes per axis: 2 (linear) or 4 (cubic)
-  int hidden;                 // 1 if some fine node is invisible (flux/I = 0 there)
+  int hidden;                 // kHidden*: whether and how some fine node is invisible
+  double gap_lo, gap_hi;      // kHiddenInterval: flux/I = 0 exactly for phases in [gap_lo, gap_hi]
   float wtg[16];              // weight of node (iT + a, ig + b) at [4 a + b]
   int run_start[kMaxRuns];    // first active pixel j (1-based) of each run
   int run_len[kMaxRuns];

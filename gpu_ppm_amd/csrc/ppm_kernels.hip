@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
     if (tid == 0) D->kept = 0;
     return;
   }
-  if (tid == 0) D->hidden = inv_min <= inv_max;
+  if (tid == 0) D->hidden = inv_min <= inv_max ? kHiddenLookup : kHiddenNone;
   if (inv_min <= inv_max) {  // gap fill (cpu/os1_cpu.cpp:152-163)
     const int beg = inv_min - 1, end = inv_max + 1;
     const double p0 = ph[beg], p1 = ph[end];
@@ -457,6 +457,27 @@ __global__ __launch_bounds__(kBlock) void k_ring_setup(DevLensing L, DevNsx N, P
     cs[nf] = cs[0];
   }
   __syncthreads();
+  if (inv_min <= inv_max) {
+    // With every node of the gap invisible and the phase table strictly
+    // ascending, flux/I interpolated at phase pp is zero exactly when
+    // ph[inv_min] <= pp <= ph[inv_max] (inside, both bracket nodes are zero
+    // or the fraction is; outside, the bracket has a visible node with a
+    // nonzero weight, up to a fraction that rounds to 1 next to the gap)
+    bool ok = true;
+    for (int k = tid; k < nf1; k += kBlock) {
+      if (k >= inv_min && k <= inv_max) ok = ok && fI[k] == 0.f;
+      if (k < nf) ok = ok && ph[k + 1] > ph[k];
+    }
+    ok = __all(ok);
+    if (lane == 0) s_int[wave] = ok;  // (s_int was last read before the gap fill's barrier)
+    __syncthreads();
+    for (int w = 0; w < kWaves; ++w) ok = ok && s_int[w] != 0;
+    if (tid == 0 && ok) {
+      D->hidden = kHiddenInterval;
+      D->gap_lo = ph[inv_min];
+      D->gap_hi = ph[inv_max];
+    }
+  }
   double zmin = 1e300, zmax = -1e300;
   for (int k = tid; k < nf1; k += kBlock) {
     zmin = fmin(zmin, (double)zf[k]);
