@@ -119,7 +119,7 @@ struct BatchState {
 // Buffers and the graph of one lane's batched launches.
 struct ppm::LaneBatch {
   ppm::GraphCache graph;
-  ppm::DevBuf d_grids, d_order, d_star_item0, d_acc, d_seg;
+  ppm::DevBuf d_grids, d_order, d_star_item0, d_acc;
 };
 
 namespace ppm {
@@ -157,8 +157,6 @@ size_t lane_device_bytes(const Lane& L, size_t* fixed) {
   *fixed = 0;
   if (L.lb) {
     for (const DevBuf* d : {&L.lb->d_grids, &L.lb->d_order, &L.lb->d_star_item0, &L.lb->d_acc}) b += d->cap;
-    *fixed = L.lb->d_seg.cap;
-    b += *fixed;
   }
   return b;
 }
@@ -167,7 +165,7 @@ void release_lane_batch(Lane& L) {
   LaneBatch* B = L.lb;
   if (!B) return;
   if (B->graph.exec) hipGraphExecDestroy(B->graph.exec);
-  for (DevBuf* b : {&B->d_grids, &B->d_order, &B->d_star_item0, &B->d_acc, &B->d_seg}) b->release();
+  for (DevBuf* b : {&B->d_grids, &B->d_order, &B->d_star_item0, &B->d_acc}) b->release();
   delete B;
   L.lb = nullptr;
 }
@@ -658,8 +656,6 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
       (e = W.evals.ensure(sizeof(unsigned long long))) ||
       (e = W.summary.ensure(sizeof(unsigned long long) * (4 * n_groups + 1))) ||
       (e = LB->d_acc.ensure(sizeof(double) * std::max<size_t>(acc_total, 1))) ||
-      (e = LB->d_seg.ensure(sizeof(double) * batch_seg_doubles() *
-                            std::max(L.per_item ? n_items : std::min(slots, n_items), 1))) ||
       (e = W.scale.ensure(sizeof(double) * n)))
     return hip_fail(c, e, "device allocation");
   std::vector<BatchGrid> bg(n_groups);
@@ -721,7 +717,6 @@ static ppm_status eval_groups_rows(ppm_ctx* c, Lane& L, std::vector<BatchGroup>&
   B.n_evals = (unsigned long long*)W.evals.p;
   B.rows_flag = (unsigned long long*)W.summary.p + 4 * n_groups;
   B.acc = (double*)LB->d_acc.p;
-  B.seg = (double*)LB->d_seg.p;
   B.n_items = n_items;
   B.n_stars = n;
   B.n_fine_max = nf_max;

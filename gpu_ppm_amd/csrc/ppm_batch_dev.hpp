@@ -74,7 +74,6 @@ struct BatchLaunch {
   int* queue;
   unsigned long long* n_evals;  // (position, energy) evaluations performed
   double* acc;             // item accumulators
-  double* seg;             // [n_wg][segment totals], workgroup scratch
   int n_items;
   int n_stars;             // stars of the launch (all grids)
   int n_fine_max;          // LDS layout: the largest grid of the launch
@@ -90,7 +89,6 @@ struct BatchLaunch {
 size_t batch_lds_bytes(int n_fine, int atmosphere, int nm, int nsx_rows, int cov_words, int ev_cap, bool fine_lds);
 int batch_threads();
 int batch_acc_copies();  // accumulator copies per item
-size_t batch_seg_doubles();  // workgroup scratch of the segment totals
 // Phase-timer totals of the batched integral (a build with PPM_PHASE_TIMERS=1;
 // false otherwise): n of the kNumPhases totals and the wavefront count.
 bool batch_phase_timers(unsigned long long* cycles, int n, bool reset);

@@ -42,7 +42,7 @@ enum BatchPhase {
   kPhSweep,     // the sweep (includes the next two)
   kPhSetup,     //   FP64 patch set-up of each batch of 64 positions
   kPhEnds,      //   window ends posted during the sweep
-  kPhOffsets,   // segment totals and offsets (both barriers)
+  kPhOffsets,   // segment offsets posted on later segments' closing events, and the closing barrier
   kPhTotal,     // the whole kernel
   kNumPhases
 };
@@ -86,15 +86,16 @@ constexpr int kMaxEPL = 6;
 constexpr int kBSets = 2;
 
 // A window end on the position axis: at pos, acc[bin] += sw * prefix.
-// pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096), and bit 31
-// (kEvInside) set when both ends of the event's window piece lie in the
-// same sweep segment.
+// pb = pos << 12 | bin (pos <= 4 N_side <= 32768, bin < 4096), and, on a
+// window's closing event (sw > 0), the sweep segment of its opening event in
+// bits 28-29 (set once the segments are known).
 struct EvRec {
   uint32_t pb;
   float sw;
 };
-constexpr uint32_t kEvInside = 1u << 31;
-__device__ inline int ev_position(uint32_t pb) { return (int)((pb >> 12) & 0x7ffffu); }
+constexpr int kEvSegShift = 28;
+__device__ inline int ev_position(uint32_t pb) { return (int)((pb >> 12) & 0xffffu); }
+__device__ inline int ev_open_segment(uint32_t pb) { return (int)((pb >> kEvSegShift) & 3u); }
 
 // Per-wavefront LDS of the sweep: the evaluation scratch and a queue of
 // covered positions (at most 63 pending + 64 new).
@@ -483,27 +484,22 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
           const int n_cov = covered_count(s_cov, n_words, lane);
           if (wp > 0) seg_lo = covered_position(s_cov, n_words, n_cov * wp / kBWP, nv, lane);
           if (wp < kBWP - 1) seg_hi = covered_position(s_cov, n_words, n_cov * (wp + 1) / kBWP, nv, lane);
-          // a window piece whose two ends lie in one segment gets the
-          // earlier segments' sum at both ends with opposite signs: its
-          // events are marked, and the offset pass below skips them
+          // each window piece's closing event records the segment of its
+          // opening event: the segments in between owe it their totals
+          // (the offset pass below)
           if (lane == 0) s_cut[wp] = seg_lo;
           __syncthreads();
           if (pr_r0 >= 0) {
-            int s0 = 0, s1 = 0;
+            int s0 = 0;
 #pragma unroll
-            for (int w = 1; w < kBWP; ++w) {
-              s0 += pr_p0 >= s_cut[w];
-              s1 += pr_p1 >= s_cut[w];
-            }
-            if (s0 == s1) {
-              s_ev[pr_r0].pb |= kEvInside;
-              s_ev[pr_r1].pb |= kEvInside;
-            }
+            for (int w = 1; w < kBWP; ++w) s0 += pr_p0 >= s_cut[w];
+            s_ev[pr_r1].pb |= (uint32_t)s0 << kEvSegShift;
           }
           __syncthreads();
         }
         const int ev_begin = wp > 0 ? events_before(s_ev, n_ev, seg_lo, lane) : 0;
         const int ev_end = wp < kBWP - 1 ? events_before(s_ev, n_ev, seg_hi, lane) : n_ev;
+        int ev_stop = ev_end;  // events taken by load_ev: this segment's, then the later ones
         PT_ADD(kPhSplit);
         double P[kBEPL];
 #pragma unroll
@@ -515,11 +511,10 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         int ev = ev_begin, win = ev_begin - 64;
         uint32_t v_pb = 0;
         float v_sw = 0.f;
-        int ev_pos = 0x7fffffff, ev_bin = 0;
-        bool ev_inside = false;
+        int ev_pos = 0x7fffffff, ev_bin = 0, ev_open = 0;
         double ev_sw = 0.0;
         auto load_ev = [&]() {
-          if (ev < ev_end) {
+          if (ev < ev_stop) {
             if (ev - win >= 64) {
               win = ev;
               if (ev + lane < n_ev) {
@@ -531,7 +526,7 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
             const uint32_t pb = (uint32_t)readlane_i((int)v_pb, ev - win);
             ev_pos = ev_position(pb);
             ev_bin = (int)(pb & 4095u);
-            ev_inside = (pb & kEvInside) != 0;
+            ev_open = ev_open_segment(pb);
             ev_sw = (double)readlane_f(v_sw, ev - win);
           } else {
             ev_pos = 0x7fffffff;
@@ -682,32 +677,27 @@ __global__ __launch_bounds__(kBThreads, 3) void k_integral_batched(DevNsx N, Bat
         while (ev < ev_end) fire();  // window ends past the last covered position
         PT_ADD(kPhSweep);
         if (kBWP > 1) {
-          // segment totals through this workgroup's global scratch, then the
-          // offset of this segment (earlier segments in order) on its events
-          double* g_T = B.seg + (size_t)blockIdx.x * kBWP * kBEPL * 64;
-          if (wp < kBWP - 1)
-#pragma unroll
-            for (int u = 0; u < kBEPL; ++u) g_T[((size_t)wp * kBEPL + u) * 64 + lane] = P[u];
-          __syncthreads();
-          if (wp > 0) {
-#pragma unroll
-            for (int u = 0; u < kBEPL; ++u) {
-              P[u] = 0.0;
-              for (int w = 0; w < wp; ++w) P[u] += g_T[((size_t)w * kBEPL + u) * 64 + lane];
-            }
-            ev = ev_begin;
-            win = ev_begin - 64;
+          // segment offsets.  A window piece opening in segment a and
+          // closing in segment b > a misses the totals of segments a..b-1
+          // (each sweep's prefix starts at zero); P now holds this segment's
+          // total, which this wavefront posts on the closing events of the
+          // later segments whose window opened here or earlier, into its own
+          // accumulator copy, with no wait for the other segments
+          if (wp < kBWP - 1) {
+            ev = ev_end;
+            win = ev_end - 64;
+            ev_stop = n_ev;
             load_ev();
-            while (ev < ev_end) {
-              if (ev_inside) {  // the window's other end is in this segment too
+            while (ev < n_ev) {
+              if (ev_sw > 0.0 && ev_open <= wp) {
+                fire();
+              } else {
                 ++ev;
                 load_ev();
-              } else {
-                fire();
               }
             }
           }
-          __syncthreads();  // the scratch is read before the next group writes it
+          __syncthreads();  // the events are read before the next group writes them
           PT_ADD(kPhOffsets);
         }
         if (g1 < C.n_pts) {
@@ -789,7 +779,6 @@ bool batch_phase_timers(unsigned long long* cycles, int n, bool reset) {
 #endif
 }
 int batch_acc_copies() { return kBWP; }
-size_t batch_seg_doubles() { return (size_t)kBWP * kMaxEPL * 64; }
 int batch_energies(int epl) { return 64 * epl; }
 
 namespace {

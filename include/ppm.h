@@ -244,8 +244,8 @@ ppm_status ppm_drain(ppm_ctx* ctx);
 /* State of the serving loop.  No counterpart in the reference (it has no
  * serving layer).  device_bytes counts the device buffers the serving lanes
  * hold (grow-only, reused by later launches); the admission budget bounds
- * them apart from each lane's segment-total scratch (fixed_bytes: up to one
- * block per resident workgroup of the device, about 9 MB). */
+ * them (fixed_bytes, memory held outside the budget, is 0: the batched
+ * kernel keeps no scratch beyond its budgeted accumulators). */
 typedef struct ppm_serve_stats {
   int64_t queued, running;   /* requests waiting, and in a launch              */
   int64_t launches[2];       /* launch sequences of the express and bulk lanes */
@@ -253,7 +253,7 @@ typedef struct ppm_serve_stats {
   size_t charged_bytes;      /* admission charges of queued and running requests */
   size_t budget_bytes;       /* the context's hbm_budget_bytes                  */
   size_t device_bytes;       /* device memory held by the serving lanes          */
-  size_t fixed_bytes;        /* of which segment-total scratch                   */
+  size_t fixed_bytes;        /* of which outside the budget (0)                   */
 } ppm_serve_stats;
 ppm_status ppm_serve_stats_get(ppm_ctx* ctx, ppm_serve_stats* out);
 
